@@ -1,0 +1,1028 @@
+// HMC chain on one GPU: Fourier-domain real state, pruned 3D FFT passes,
+// fused leapfrog updates, CSR kriging products, float64 energies.
+//
+// State layout in HBM (working precision T):
+//   muA, muB  : half spectra of mu (current, proposal)  [M2][M1][M0/2+1]
+//   p         : half spectrum of the momentum
+//   miA, miB, pim : imaginary chain (DR_STATE_COMPAT only)
+//   invlam    : 1/lambda on the half spectrum, 0 on Z
+//   T1, T2    : pass scratch (after z / after z,y of the inverse)
+//   RA, RB    : mu on the image region in real space, [n2][n1][n0]
+//   Yg, mask, WTv : data on the image region
+//   W (CSR, columns = image-region grid index), WT (transposed CSR)
+// Reference: include/dualres/MultiResParameters.h (whole file).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <type_traits>
+#include <vector>
+
+#include "dr_kernels.hpp"
+
+namespace drh {
+
+// ---------------------------------------------------------------- small utils
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+std::vector<int> factorize(int M) {
+  std::vector<int> f;
+  int m = M;
+  while (m % 8 == 0 && m > 8) { f.push_back(8); m /= 8; }
+  if (m % 8 == 0) { f.push_back(8); m /= 8; }
+  while (m % 4 == 0) { f.push_back(4); m /= 4; }
+  while (m % 2 == 0) { f.push_back(2); m /= 2; }
+  for (int r : {3, 5, 7}) while (m % r == 0) { f.push_back(r); m /= r; }
+  if (m != 1) throw Error(DR_EDOMAIN, "FFT length " + std::to_string(M) + " has a prime factor > 7");
+  return f;
+}
+
+int pick_tbs(int len, int rows_lds, int elem_bytes, bool padded, int max_lines, int tile_elems) {
+  int tbs = 0;
+  auto lds = [&](int tb) { return (long)rows_lds * (padded ? tb + 1 : tb) * elem_bytes; };
+  while (true) {
+    const int tb = 2 << tbs;
+    if ((long)tb * len > tile_elems) break;
+    if (lds(tb) > 63488) break;
+    if ((1 << tbs) >= max_lines) break;
+    ++tbs;
+  }
+  return tbs;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(DR_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  void zero(hipStream_t s) {
+    if (p) DR_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+__global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a, const double* part, double* sums) {
+  __shared__ double red[kThreads];
+  const int s = blockIdx.x;
+  if (s >= a.nslot) return;
+  const int nb = a.nblk[s];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) acc += part[s * a.pstride + i];
+  const double r = block_sum(acc, red);
+  if (threadIdx.x == 0) sums[s] = nb > 0 ? r : 0.0;
+}
+
+// twiddle table exp(-2 pi i e / M), e in [0, M), rounded from long double
+template <typename T>
+static std::vector<cx<T>> twiddles(int M) {
+  std::vector<cx<T>> t(M);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int e = 0; e < M; ++e) {
+    const long double a = -2.0L * pi * (long double)e / (long double)M;
+    t[e] = mk<T>((T)std::cos(a), (T)std::sin(a));
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------- reduction slots
+enum Slot {
+  S_E0 = 0, S_E0I, S_PRIOR_A, S_PRIORI_A, S_SSEH_A, S_SSES_A,
+  S_PRIOR_B, S_PRIORI_B, S_KIN_B, S_KINI_B, S_SSEH_B, S_SSES_B,
+  S_KIN_A, S_KINI_A, S_ZCOUNT, S_TMP, NSLOT
+};
+
+// ---------------------------------------------------------------- chain
+struct ChainBase {
+  virtual ~ChainBase() = default;
+  dr_config cfg{};
+  int M[3] = {0, 0, 0};
+  int H0 = 0;
+  long N = 0, Nh = 0;
+  int reg[3] = {0, 0, 0};
+  long nreg = 0;
+  long n_h = 0, n_s = 0;
+  std::vector<int32_t> ijk;         // data indices (n_h x 3)
+  std::vector<int32_t> out_ijk;     // output indices (n_out x 3)
+  std::vector<double> Yh, Ys;       // data in nonzero order
+  std::vector<int64_t> W_rp;        // CSR as given (columns index Yh)
+  std::vector<int32_t> W_cols;
+  std::vector<double> W_vals;
+  double sigma[2] = {1.0, 1.0};
+  uint64_t iteration = 0;
+  long n_nonpos = 0;
+  double max_eig = 0.0;
+  bool data_set = false;
+  // cached energies of the current state A
+  bool stats_valid = false;
+  double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
+  // profiling
+  bool prof = false;
+  double prof_ms = 0.0, prof_fft_ms = 0.0, prof_w_ms = 0.0;
+  long prof_n = 0;
+  hipStream_t stream = nullptr;
+
+  virtual void build() = 0;
+  virtual void upload_data() = 0;
+  virtual void add_data_to_mu() = 0;
+  virtual void set_region(const int r[3]) = 0;
+  virtual void get_mu_region_gather(const std::vector<int32_t>& gidx, double* out) = 0;
+  virtual void update(double eps, int L, double u, const double* zr, const double* zi, dr_update_result* res) = 0;
+  virtual void leapfrog(double eps, int L, dr_update_result* res) = 0;
+  virtual void set_state(const double* mr, const double* mi, const double* pr, const double* pi) = 0;
+  virtual void get_state(double* mr, double* mi, double* pr, double* pi) = 0;
+  virtual void gradient(double* gr, double* gi) = 0;
+  virtual void refresh_stats() = 0;
+  virtual void get_lambda(double* out) = 0;
+
+  long region_index(int x, int y, int z) const { return x + (long)reg[0] * (y + (long)reg[1] * z); }
+  double log_lik(double sh, double ss) const {
+    double ll = sigma[0] * sh;
+    if (cfg.n_datasets == 2) ll += sigma[1] * ss;
+    return -0.5 * ll;
+  }
+  bool compat() const { return cfg.state_mode == DR_STATE_COMPAT; }
+};
+
+template <typename T>
+struct Chain : public ChainBase {
+  // FFT tables and pass descriptors (x: length M0/2 into the M0 table)
+  DevBuf<cx<float>> twf[3];
+  DevBuf<cx<double>> twd[3];
+  AxisFFT<float> adf[3]{};
+  AxisFFT<double> add[3]{};
+  // state
+  DevBuf<cx<T>> muA, muB, p, miA, miB, pim, T1, T2;
+  DevBuf<T> invlam;
+  DevBuf<double> lam;  // Re lambda on the half spectrum (float64)
+  DevBuf<T> RA, RB, Yg, WTv, Rf, zbuf;
+  DevBuf<uint8_t> mask;
+  DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
+  DevBuf<T> Wvals, WTvals, Ysd, vsd, outbuf;
+  DevBuf<double> part, sums;
+  long pstride = 0;
+  int slot_nblk[NSLOT] = {0};
+  std::vector<int32_t> out_g;  // region grid index of each output voxel
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  ~Chain() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // ------------------------------------------------------------ launch helpers
+  template <typename U>
+  const AxisFFT<U>& axis_dev(int a) const {
+    if constexpr (std::is_same<U, float>::value) return adf[a];
+    else return add[a];
+  }
+
+  // transform along z (a=2) or y (a=1) over lines with contiguous batch NB
+  template <typename U, bool INV, class Pro, class Epi>
+  void strided(int a, int n_outer, int NB, int k_in, int k_out, Pro pro, Epi epi, int slot = -1) {
+    const AxisFFT<U> ad = axis_dev<U>(a);
+    DR_REQUIRE(k_in >= 1 && k_in <= ad.M && k_out >= 1 && k_out <= ad.M && NB >= 1 && n_outer >= 1,
+               DR_ELOGIC, "strided pass: bad extents");
+    const int tile = sizeof(U) == 4 ? kTileElems : kTileElems;
+    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, tile);
+    const int TB = 1 << tbs;
+    dim3 grid(cdiv(NB, TB), n_outer);
+    const size_t lds = (size_t)ad.M * TB * sizeof(cx<U>);
+    if (slot >= 0) note_blocks(slot, grid.x * grid.y);
+    hipLaunchKernelGGL((k_strided<U, INV, Pro, Epi>), grid, dim3(kThreads), lds, stream, ad, NB, k_in, k_out, tbs,
+                       pro, epi);
+    DR_HIP(hipGetLastError());
+  }
+  template <typename U, class Pro, class Epi>
+  void xr2c(int nrows, Pro pro, Epi epi, int slot = -1) {
+    const AxisFFT<U> ad = axis_dev<U>(0);
+    const int Mh = ad.M;
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, nrows, kTileElems);
+    const int TB = 1 << tbs;
+    const int grid = cdiv(nrows, TB);
+    const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
+    if (slot >= 0) note_blocks(slot, grid);
+    hipLaunchKernelGGL((k_r2c_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
+    DR_HIP(hipGetLastError());
+  }
+  template <typename U, class Pro, class Epi>
+  void xc2r(int nrows, Pro pro, Epi epi, int slot = -1) {
+    const AxisFFT<U> ad = axis_dev<U>(0);
+    const int Mh = ad.M;
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, nrows, kTileElems);
+    const int TB = 1 << tbs;
+    const int grid = cdiv(nrows, TB);
+    const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
+    if (slot >= 0) note_blocks(slot, grid);
+    hipLaunchKernelGGL((k_c2r_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
+    DR_HIP(hipGetLastError());
+  }
+  void note_blocks(int slot, long nb) {
+    DR_REQUIRE(nb <= pstride, DR_ELOGIC, "reduction grid exceeds partial buffer");
+    slot_nblk[slot] = (int)nb;
+  }
+  double* pslot(int s) { return part.p + (long)s * pstride; }
+
+  // ------------------------------------------------------------ build
+  void build() override {
+    DR_HIP(hipSetDevice(cfg.device));
+    DR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) DR_HIP(hipEventCreate(&e));
+    H0 = M[0] / 2 + 1;
+    N = (long)M[0] * M[1] * M[2];
+    Nh = (long)H0 * M[1] * M[2];
+    for (int a = 0; a < 3; ++a) {
+      const int len = M[a];
+      const int flen = a == 0 ? len / 2 : len;
+      const int tws = a == 0 ? 2 : 1;
+      auto td = twiddles<double>(len);
+      twd[a].alloc(len);
+      DR_HIP(hipMemcpy(twd[a].p, td.data(), len * sizeof(cx<double>), hipMemcpyHostToDevice));
+      add[a] = make_axis<double>(flen, twd[a].p, tws);
+      if (std::is_same<T, float>::value) {
+        auto tf = twiddles<float>(len);
+        twf[a].alloc(len);
+        DR_HIP(hipMemcpy(twf[a].p, tf.data(), len * sizeof(cx<float>), hipMemcpyHostToDevice));
+        adf[a] = make_axis<float>(flen, twf[a].p, tws);
+      }
+    }
+    // partial buffer: the largest grid any reducing kernel uses
+    pstride = std::max<long>(1024, Nh / 8 + 1);
+    pstride = std::max<long>(pstride, (long)M[1] * M[2] + 1);
+    pstride = std::max<long>(pstride, (long)M[1] * H0 + 1);
+    part.alloc((size_t)NSLOT * pstride);
+    sums.alloc(NSLOT);
+    part.zero(stream);
+    sums.zero(stream);
+    invlam.alloc(Nh);
+    lam.alloc(Nh);
+    muA.alloc(Nh);
+    muB.alloc(Nh);
+    p.alloc(Nh);
+    T1.alloc(Nh);
+    T2.alloc(Nh);
+    if (compat()) {
+      miA.alloc(Nh);
+      miB.alloc(Nh);
+      pim.alloc(Nh);
+    }
+    int r[3];
+    for (int a = 0; a < 3; ++a) {
+      int m = 0;
+      for (long i = 0; i < n_h; ++i) m = std::max(m, ijk[i * 3 + a]);
+      r[a] = m + 1;
+    }
+    reg[0] = reg[1] = reg[2] = 0;
+    set_region(r);
+    compute_spectrum();
+    init_mu();
+  }
+
+  // ------------------------------------------------------------ spectrum
+  void compute_spectrum() {
+    // float64 scratch for the spectrum transform
+    DevBuf<cx<double>> s1, s2;
+    s1.alloc(Nh);
+    s2.alloc(Nh);
+    LoadBase lb{};
+    lb.M0 = M[0]; lb.M1 = M[1]; lb.M2 = M[2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) lb.q[3 * i + j] = cfg.qform[4 * i + j];
+    lb.tau2 = cfg.theta[0]; lb.psi = cfg.theta[1]; lb.nu = cfg.theta[2]; lb.covfun = cfg.covfun;
+    const int rows = M[1] * M[2];
+    xr2c<double>(rows, lb, StoreRowC<double>{{}, s2.p, H0});
+    strided<double, false>(1, M[2], H0, M[1], M[1], LoadS<double>{{}, s2.p, (long)M[1] * H0, H0},
+                           StoreS<double>{{}, s1.p, (long)M[1] * H0, H0});
+    EpiSpectrum<T> es{lam.p, invlam.p, (long)M[1] * H0, H0, pslot(S_ZCOUNT), 0.0};
+    strided<double, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<double>{{}, s1.p, 0, (long)M[1] * H0}, es, S_ZCOUNT);
+    finalize();
+    double h[NSLOT];
+    DR_HIP(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    double l0 = 0.0;
+    DR_HIP(hipMemcpyAsync(&l0, lam.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    n_nonpos = (long)llround(h[S_ZCOUNT]);
+    max_eig = l0 / (double)N;
+    if (n_nonpos >= N / 2) throw Error(DR_EDOMAIN, "Too many negative eigen values");
+  }
+
+  void get_lambda(double* out) override {
+    std::vector<double> h(Nh);
+    DR_HIP(hipMemcpyAsync(h.data(), lam.p, Nh * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    // expand the Hermitian half spectrum (lambda even and real)
+    for (int z = 0; z < M[2]; ++z)
+      for (int y = 0; y < M[1]; ++y)
+        for (int x = 0; x < M[0]; ++x) {
+          int kx = x, ky = y, kz = z;
+          if (x >= H0) { kx = M[0] - x; ky = (M[1] - y) % M[1]; kz = (M[2] - z) % M[2]; }
+          out[x + (long)M[0] * (y + (long)M[1] * z)] = h[kx + (long)H0 * (ky + (long)M[1] * kz)];
+        }
+  }
+
+  // ------------------------------------------------------------ generic transforms
+  // full real torus (Rf) -> half spectrum X
+  void r2c_full(const T* src, cx<T>* X) {
+    xr2c<T>(M[1] * M[2], LoadRowR<T>{{}, src, M[0]}, StoreRowC<T>{{}, T2.p, H0});
+    strided<T, false>(1, M[2], H0, M[1], M[1], LoadS<T>{{}, T2.p, (long)M[1] * H0, H0},
+                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
+    strided<T, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0},
+                      StoreS<T>{{}, X, 0, (long)M[1] * H0});
+  }
+  // half spectrum X -> full real torus dst, scaled by 1/N
+  void c2r_full(const cx<T>* X, T* dst) {
+    strided<T, true>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, X, 0, (long)M[1] * H0},
+                     StoreS<T>{{}, T1.p, 0, (long)M[1] * H0});
+    strided<T, true>(1, M[2], H0, M[1], M[1], LoadS<T>{{}, T1.p, (long)M[1] * H0, H0},
+                     StoreS<T>{{}, T2.p, (long)M[1] * H0, H0});
+    xc2r<T>(M[1] * M[2], LoadRowC<T>{{}, T2.p, H0}, StoreRowR<T>{{}, dst, M[0], (T)(1.0 / (double)N)});
+  }
+  // inverse transform pruned to the image region; Pro is the z-pass prologue
+  template <class Pro>
+  void c2r_region(Pro pro, T* R) {
+    strided<T, true>(2, 1, M[1] * H0, M[2], reg[2], pro, StoreS<T>{{}, T1.p, 0, (long)M[1] * H0});
+    strided<T, true>(1, reg[2], H0, M[1], reg[1], LoadS<T>{{}, T1.p, (long)M[1] * H0, H0},
+                     StoreS<T>{{}, T2.p, (long)reg[1] * H0, H0});
+    xc2r<T>(reg[1] * reg[2], LoadRowC<T>{{}, T2.p, H0}, StoreRowR<T>{{}, R, reg[0], (T)(1.0 / (double)N)});
+  }
+  // forward transform from the image region; Pro is the x-pass prologue,
+  // Epi the z-pass epilogue
+  template <class Pro, class Epi>
+  void r2c_region(Pro pro, Epi epi, int pro_slot, int epi_slot) {
+    xr2c<T>(reg[1] * reg[2], pro, StoreRowC<T>{{}, T2.p, H0}, pro_slot);
+    strided<T, false>(1, reg[2], H0, reg[1], M[1], LoadS<T>{{}, T2.p, (long)reg[1] * H0, H0},
+                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
+    strided<T, false>(2, 1, M[1] * H0, reg[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0}, epi, epi_slot);
+  }
+  // full-torus noise field -> spectral momentum (or initial mu)
+  template <class Epi>
+  void noise_r2c(uint64_t stream_id, const T* zinj, int slot, Epi epi) {
+    if (zinj) {
+      xr2c<T>(M[1] * M[2], LoadNoiseBuf<T>{zinj, M[0], pslot(slot), 0.0}, StoreRowC<T>{{}, T2.p, H0}, slot);
+    } else {
+      xr2c<T>(M[1] * M[2], LoadNoise<T>{cfg.seed, stream_id, M[0] / 2, pslot(slot), 0.0},
+              StoreRowC<T>{{}, T2.p, H0}, slot);
+    }
+    strided<T, false>(1, M[2], H0, M[1], M[1], LoadS<T>{{}, T2.p, (long)M[1] * H0, H0},
+                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
+    strided<T, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0}, epi);
+  }
+
+  void finalize() {
+    FinalizeArgs a{};
+    a.nslot = NSLOT;
+    for (int s = 0; s < NSLOT; ++s) a.nblk[s] = slot_nblk[s];
+    a.pstride = pstride;
+    hipLaunchKernelGGL(k_finalize, dim3(NSLOT), dim3(kThreads), 0, stream, a, part.p, sums.p);
+    DR_HIP(hipGetLastError());
+  }
+  void read_sums(double* h) {
+    DR_HIP(hipMemcpyAsync(h, sums.p, NSLOT * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+  }
+  void clear_slots() {
+    for (int s = 0; s < NSLOT; ++s) slot_nblk[s] = 0;
+  }
+
+  // ------------------------------------------------------------ init
+  void init_mu() {
+    // mu = B(F(z) sqrt(lambda) / (10 N)) with z complex normal (:743-759)
+    noise_r2c(2, nullptr, S_TMP, EpiInitMu<T>{{}, muA.p, lam.p, (long)M[1] * H0});
+    if (compat()) noise_r2c(3, nullptr, S_TMP, EpiInitMu<T>{{}, miA.p, lam.p, (long)M[1] * H0});
+    stats_valid = false;
+  }
+
+  // ------------------------------------------------------------ region / data
+  void set_region(const int r[3]) override {
+    for (int a = 0; a < 3; ++a)
+      DR_REQUIRE(r[a] >= 1 && r[a] <= M[a], DR_EDOMAIN, "image region exceeds the embedding torus");
+    if (r[0] == reg[0] && r[1] == reg[1] && r[2] == reg[2]) return;
+    for (int a = 0; a < 3; ++a) reg[a] = r[a];
+    nreg = (long)reg[0] * reg[1] * reg[2];
+    RA.alloc(nreg);
+    RB.alloc(nreg);
+    Yg.alloc(nreg);
+    mask.alloc(nreg);
+    Yg.zero(stream);
+    mask.zero(stream);
+    std::vector<int32_t> g(n_h);
+    for (long i = 0; i < n_h; ++i) g[i] = (int32_t)region_index(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+    gI.alloc(n_h);
+    DR_HIP(hipMemcpyAsync(gI.p, g.data(), n_h * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    if (data_set) upload_data();
+    if (!out_ijk.empty()) {
+      out_g.resize(out_ijk.size() / 3);
+      for (size_t i = 0; i < out_g.size(); ++i)
+        out_g[i] = (int32_t)region_index(out_ijk[3 * i], out_ijk[3 * i + 1], out_ijk[3 * i + 2]);
+    }
+    stats_valid = false;
+  }
+
+  void upload_data() override {
+    // data on the region grid
+    std::vector<T> yg(nreg, (T)0);
+    std::vector<uint8_t> mk_(nreg, 0);
+    std::vector<int32_t> g(n_h);
+    for (long i = 0; i < n_h; ++i) {
+      g[i] = (int32_t)region_index(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+      yg[g[i]] = (T)Yh[i];
+      mk_[g[i]] = 1;
+    }
+    DR_HIP(hipMemcpyAsync(Yg.p, yg.data(), nreg * sizeof(T), hipMemcpyHostToDevice, stream));
+    DR_HIP(hipMemcpyAsync(mask.p, mk_.data(), nreg, hipMemcpyHostToDevice, stream));
+    if (cfg.n_datasets == 2) {
+      const long need = cdiv(n_s, kThreads / 64) + 1;
+      if (need > pstride) {
+        pstride = need;
+        part.alloc((size_t)NSLOT * pstride);
+        part.zero(stream);
+      }
+      const long nnz = W_rp[n_s];
+      DR_REQUIRE(nnz < (1L << 31), DR_EDOMAIN, "W has too many nonzeros for 32-bit indices");
+      std::vector<int32_t> rp(n_s + 1), cols(nnz);
+      std::vector<T> vals(nnz);
+      std::vector<int32_t> trp(n_h + 1, 0), trows(nnz);
+      std::vector<T> tvals(nnz);
+      for (long i = 0; i <= n_s; ++i) rp[i] = (int32_t)W_rp[i];
+      for (long j = 0; j < nnz; ++j) {
+        const int32_t c = W_cols[j];
+        DR_REQUIRE(c >= 0 && c < n_h, DR_EDOMAIN, "W column index out of range");
+        cols[j] = g[c];
+        vals[j] = (T)W_vals[j];
+        trp[c + 1]++;
+      }
+      for (long d = 0; d < n_h; ++d) trp[d + 1] += trp[d];
+      std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+      for (long i = 0; i < n_s; ++i)
+        for (long j = W_rp[i]; j < W_rp[i + 1]; ++j) {
+          const int32_t c = W_cols[j];
+          trows[fill[c]] = (int32_t)i;
+          tvals[fill[c]] = (T)W_vals[j];
+          fill[c]++;
+        }
+      Wrp.alloc(n_s + 1);
+      Wcols.alloc(std::max<long>(nnz, 1));
+      Wvals.alloc(std::max<long>(nnz, 1));
+      WTrp.alloc(n_h + 1);
+      WTrows.alloc(std::max<long>(nnz, 1));
+      WTvals.alloc(std::max<long>(nnz, 1));
+      Ysd.alloc(n_s);
+      vsd.alloc(n_s);
+      WTv.alloc(nreg);
+      WTv.zero(stream);
+      std::vector<T> ys(n_s);
+      for (long i = 0; i < n_s; ++i) ys[i] = (T)Ys[i];
+      DR_HIP(hipMemcpyAsync(Wrp.p, rp.data(), (n_s + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      if (nnz) {
+        DR_HIP(hipMemcpyAsync(Wcols.p, cols.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        DR_HIP(hipMemcpyAsync(Wvals.p, vals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
+        DR_HIP(hipMemcpyAsync(WTrows.p, trows.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        DR_HIP(hipMemcpyAsync(WTvals.p, tvals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
+      }
+      DR_HIP(hipMemcpyAsync(WTrp.p, trp.data(), (n_h + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      DR_HIP(hipMemcpyAsync(Ysd.p, ys.data(), n_s * sizeof(T), hipMemcpyHostToDevice, stream));
+    }
+    DR_HIP(hipStreamSynchronize(stream));
+    data_set = true;
+    stats_valid = false;
+  }
+
+  void add_data_to_mu() override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "add_data_to_mu: data not set");
+    r2c_region(LoadRowR<T>{{}, Yg.p, reg[0]}, EpiAddTo<T>{{}, muA.p, (long)M[1] * H0}, -1, -1);
+    stats_valid = false;
+  }
+
+  void get_mu_region_gather(const std::vector<int32_t>& g, double* out) override {
+    refresh_stats();
+    const long n = (long)g.size();
+    if (n == 0) return;
+    DevBuf<int> gd;
+    gd.alloc(n);
+    outbuf.alloc(n);
+    DR_HIP(hipMemcpyAsync(gd.p, g.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, stream, n, gd.p, RA.p, outbuf.p);
+    DR_HIP(hipGetLastError());
+    std::vector<T> h(n);
+    DR_HIP(hipMemcpyAsync(h.data(), outbuf.p, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    for (long i = 0; i < n; ++i) out[i] = (double)h[i];
+  }
+
+  // ------------------------------------------------------------ gradient pieces
+  // likelihood products at region field R: v = sigma_s (Ys - W R), WTv = W^T v
+  void w_products(const T* R, int slot) {
+    if (cfg.n_datasets != 2) return;
+    const int g1 = cdiv(n_s, kThreads / 64);
+    if (slot >= 0) note_blocks(slot, g1);
+    hipLaunchKernelGGL(k_wmu<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, Wrp.p, Wcols.p, Wvals.p, R, Ysd.p,
+                       (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr);
+    DR_HIP(hipGetLastError());
+    const int g2 = cdiv(n_h, kThreads / 16);
+    hipLaunchKernelGGL(k_wtv<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, WTrp.p, WTrows.p, WTvals.p, vsd.p,
+                       gI.p, WTv.p);
+    DR_HIP(hipGetLastError());
+  }
+  LoadResidual<T> residual(const T* R, int slot) {
+    return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr, (T)sigma[0], reg[0],
+                           slot >= 0 ? pslot(slot) : nullptr, 0.0};
+  }
+  // gradient at the state held in X (real space result R), then p += coef * g
+  void grad_step(const cx<T>* X, const cx<T>* Xi, T* R, T coef, bool red, int s_sh, int s_ss, int s_prior,
+                 int s_priori, int s_kin, int s_kini) {
+    w_products(R, red ? s_ss : -1);
+    EpiMomentum<T> em{X, p.p, invlam.p, (long)M[1] * H0, coef, (T)sigma[0], H0, Xi,
+                      compat() ? pim.p : nullptr, red ? 1 : 0, part.p, pstride, s_prior, s_priori, s_kin, s_kini,
+                      0.0, 0.0, 0.0, 0.0};
+    if (red) {
+      // the epilogue writes four slots from the same grid
+      const int NB = M[1] * H0;
+      const int tbs = pick_tbs(M[2], M[2], sizeof(cx<T>), false, NB, kTileElems);
+      const long nb = cdiv(NB, 1 << tbs);
+      note_blocks(s_prior, nb);
+      note_blocks(s_priori, nb);
+      note_blocks(s_kin, nb);
+      note_blocks(s_kini, nb);
+    }
+    r2c_region(residual(R, red ? s_sh : -1), em, red ? s_sh : -1, -1);
+  }
+
+  // ------------------------------------------------------------ HMC iteration
+  void leapfrog_core(double eps, int L, bool start_red) {
+    const long sk = (long)M[1] * H0;
+    // gradient at A and half step
+    c2r_region(LoadS<T>{{}, muA.p, 0, sk}, RA.p);
+    grad_step(muA.p, compat() ? miA.p : nullptr, RA.p, (T)(0.5 * eps), start_red, S_SSEH_A, S_SSES_A, S_PRIOR_A,
+              S_PRIORI_A, S_TMP, S_TMP);
+    for (int s = 0; s < L; ++s) {
+      const double k = (s == L - 1) ? 0.5 : 1.0;
+      const cx<T>* src = s == 0 ? muA.p : muB.p;
+      const cx<T>* srci = compat() ? (s == 0 ? miA.p : miB.p) : nullptr;
+      LoadPosUpdate<T> pu{{}, src, muB.p, p.p, invlam.p, sk, (T)eps, (T)sigma[0], srci,
+                          compat() ? miB.p : nullptr, compat() ? pim.p : nullptr};
+      if (prof && L >= 2 && s == 1) DR_HIP(hipEventRecord(ev[0], stream));
+      c2r_region(pu, RB.p);
+      grad_step(muB.p, compat() ? miB.p : nullptr, RB.p, (T)(k * eps), s == L - 1, S_SSEH_B, S_SSES_B, S_PRIOR_B,
+                S_PRIORI_B, S_KIN_B, S_KINI_B);
+      if (prof && L >= 2 && s == L - 1) DR_HIP(hipEventRecord(ev[1], stream));
+    }
+  }
+
+  void accept_swap() {
+    std::swap(muA.p, muB.p);
+    std::swap(RA.p, RB.p);
+    if (compat()) std::swap(miA.p, miB.p);
+  }
+
+  void update(double eps, int L, double u, const double* zr, const double* zi, dr_update_result* res) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "update: data not set");
+    DR_REQUIRE(L >= 1, DR_EDOMAIN, "update: L must be >= 1");
+    clear_slots();
+    const long sk = (long)M[1] * H0;
+    // momentum p = sqrt(lambda_m) F(z)
+    const uint64_t it = iteration++;
+    if (zr) upload_real(zr, zbuf);
+    noise_r2c(4 * it + 0, zr ? zbuf.p : nullptr, S_E0, EpiNoiseMomentum<T>{{}, p.p, invlam.p, sk, (T)sigma[0]});
+    if (compat()) {
+      if (zr) upload_real(zi, zbuf);
+      noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I, EpiNoiseMomentum<T>{{}, pim.p, invlam.p, sk, (T)sigma[0]});
+    }
+    leapfrog_core(eps, L, true);
+    finalize();
+    double h[NSLOT];
+    read_sums(h);
+    if (prof && L >= 2) accumulate_profile(L - 1);
+    const double nN = (double)N;
+    const double e0 = -0.5 * (h[S_E0] + (compat() ? h[S_E0I] : 0.0));
+    const double lp0 = -0.5 * (h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0)) / nN;
+    const double ll0 = log_lik(h[S_SSEH_A], h[S_SSES_A]);
+    const double lp1 = -0.5 * (h[S_PRIOR_B] + (compat() ? h[S_PRIORI_B] : 0.0)) / nN;
+    const double ll1 = log_lik(h[S_SSEH_B], h[S_SSES_B]);
+    const double k1 = -0.5 * (h[S_KIN_B] + (compat() ? h[S_KINI_B] : 0.0)) / nN;
+    const double H0e = e0 + lp0 + ll0;
+    const double H1e = lp1 + ll1 + k1;
+    double R = std::exp(H1e - H0e);
+    if (std::isnan(R)) R = 0.0;
+    const bool acc = u < R;
+    if (acc) {
+      accept_swap();
+      prior_sum = h[S_PRIOR_B] + (compat() ? h[S_PRIORI_B] : 0.0);
+      sse_h = h[S_SSEH_B];
+      sse_s = h[S_SSES_B];
+    } else {
+      prior_sum = h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0);
+      sse_h = h[S_SSEH_A];
+      sse_s = h[S_SSES_A];
+    }
+    stats_valid = true;
+    if (res) {
+      res->R = R;
+      res->mh_rate = std::min(1.0, R);
+      res->H0 = H0e;
+      res->H1 = H1e;
+      res->log_prior0 = lp0;
+      res->log_lik0 = ll0;
+      res->e0 = e0;
+      res->log_prior1 = lp1;
+      res->log_lik1 = ll1;
+      res->k1 = k1;
+      res->sse_h = sse_h;
+      res->sse_s = sse_s;
+      res->accepted = acc ? 1 : 0;
+    }
+  }
+
+  void leapfrog(double eps, int L, dr_update_result* res) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "leapfrog: data not set");
+    DR_REQUIRE(L >= 1, DR_EDOMAIN, "leapfrog: L must be >= 1");
+    clear_slots();
+    quad(p.p, S_KIN_A, (T)sigma[0]);
+    if (compat()) quad(pim.p, S_KINI_A, (T)sigma[0]);
+    leapfrog_core(eps, L, true);
+    finalize();
+    double h[NSLOT];
+    read_sums(h);
+    if (prof && L >= 2) accumulate_profile(L - 1);
+    const double nN = (double)N;
+    accept_swap();
+    prior_sum = h[S_PRIOR_B] + (compat() ? h[S_PRIORI_B] : 0.0);
+    sse_h = h[S_SSEH_B];
+    sse_s = h[S_SSES_B];
+    stats_valid = true;
+    if (res) {
+      res->e0 = -0.5 * (h[S_KIN_A] + (compat() ? h[S_KINI_A] : 0.0)) / nN;
+      res->log_prior0 = -0.5 * (h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0)) / nN;
+      res->log_lik0 = log_lik(h[S_SSEH_A], h[S_SSES_A]);
+      res->log_prior1 = -0.5 * prior_sum / nN;
+      res->log_lik1 = log_lik(sse_h, sse_s);
+      res->k1 = -0.5 * (h[S_KIN_B] + (compat() ? h[S_KINI_B] : 0.0)) / nN;
+      res->H0 = res->e0 + res->log_prior0 + res->log_lik0;
+      res->H1 = res->k1 + res->log_prior1 + res->log_lik1;
+      res->R = std::exp(res->H1 - res->H0);
+      res->mh_rate = std::min(1.0, res->R);
+      res->sse_h = sse_h;
+      res->sse_s = sse_s;
+      res->accepted = 1;
+    }
+  }
+
+  void quad(const cx<T>* X, int slot, T sig) {
+    const int g = (int)std::min<long>(cdiv(Nh, kThreads), 2048);
+    note_blocks(slot, g);
+    hipLaunchKernelGGL(k_quad<T>, dim3(g), dim3(kThreads), 0, stream, Nh, H0, X, invlam.p, sig, pslot(slot));
+    DR_HIP(hipGetLastError());
+  }
+
+  void refresh_stats() override {
+    if (stats_valid) return;
+    clear_slots();
+    const long sk = (long)M[1] * H0;
+    c2r_region(LoadS<T>{{}, muA.p, 0, sk}, RA.p);
+    quad(muA.p, S_PRIOR_A, (T)-1);
+    if (compat()) quad(miA.p, S_PRIORI_A, (T)-1);
+    if (data_set) {
+      const int g = (int)std::min<long>(cdiv(nreg, kThreads), 2048);
+      note_blocks(S_SSEH_A, g);
+      hipLaunchKernelGGL(k_sse_h<T>, dim3(g), dim3(kThreads), 0, stream, nreg, RA.p, Yg.p, mask.p, pslot(S_SSEH_A));
+      DR_HIP(hipGetLastError());
+      w_products(RA.p, S_SSES_A);
+    }
+    finalize();
+    double h[NSLOT];
+    read_sums(h);
+    prior_sum = h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0);
+    sse_h = h[S_SSEH_A];
+    sse_s = cfg.n_datasets == 2 ? h[S_SSES_A] : 0.0;
+    stats_valid = true;
+  }
+
+  // ------------------------------------------------------------ test hooks
+  void upload_real(const double* src, DevBuf<T>& dst) {
+    if (dst.n != (size_t)N) dst.alloc(N);
+    std::vector<T> h(N);
+    for (long i = 0; i < N; ++i) h[i] = (T)src[i];
+    DR_HIP(hipMemcpyAsync(dst.p, h.data(), N * sizeof(T), hipMemcpyHostToDevice, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+  }
+  void download_real(const DevBuf<T>& src, double* dst) {
+    std::vector<T> h(N);
+    DR_HIP(hipMemcpyAsync(h.data(), src.p, N * sizeof(T), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    for (long i = 0; i < N; ++i) dst[i] = (double)h[i];
+  }
+  void set_state(const double* mr, const double* mi, const double* pr, const double* pi) override {
+    if (mr) { upload_real(mr, Rf); r2c_full(Rf.p, muA.p); }
+    if (pr) { upload_real(pr, Rf); r2c_full(Rf.p, p.p); }
+    if (compat()) {
+      if (mi) { upload_real(mi, Rf); r2c_full(Rf.p, miA.p); }
+      if (pi) { upload_real(pi, Rf); r2c_full(Rf.p, pim.p); }
+    }
+    DR_HIP(hipStreamSynchronize(stream));
+    stats_valid = false;
+  }
+  void get_state(double* mr, double* mi, double* pr, double* pi) override {
+    if (Rf.n != (size_t)N) Rf.alloc(N);
+    if (mr) { c2r_full(muA.p, Rf.p); download_real(Rf, mr); }
+    if (pr) { c2r_full(p.p, Rf.p); download_real(Rf, pr); }
+    if (mi) {
+      if (compat()) { c2r_full(miA.p, Rf.p); download_real(Rf, mi); }
+      else std::fill(mi, mi + N, 0.0);
+    }
+    if (pi) {
+      if (compat()) { c2r_full(pim.p, Rf.p); download_real(Rf, pi); }
+      else std::fill(pi, pi + N, 0.0);
+    }
+  }
+  void gradient(double* gr, double* gi) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "gradient: data not set");
+    if (Rf.n != (size_t)N) Rf.alloc(N);
+    const long sk = (long)M[1] * H0;
+    c2r_region(LoadS<T>{{}, muA.p, 0, sk}, RA.p);
+    w_products(RA.p, -1);
+    r2c_region(residual(RA.p, -1), EpiGrad<T>{{}, muA.p, muB.p, invlam.p, sk}, -1, -1);
+    c2r_full(muB.p, Rf.p);
+    download_real(Rf, gr);
+    if (gi) {
+      if (compat()) {
+        DR_HIP(hipMemcpyAsync(muB.p, miA.p, Nh * sizeof(cx<T>), hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(k_scale_spec<T>, dim3(cdiv(Nh, kThreads)), dim3(kThreads), 0, stream, Nh, muB.p, invlam.p,
+                           (T)-1);
+        DR_HIP(hipGetLastError());
+        c2r_full(muB.p, Rf.p);
+        download_real(Rf, gi);
+      } else {
+        std::fill(gi, gi + N, 0.0);
+      }
+    }
+  }
+
+  void accumulate_profile(long n) {
+    float ms = 0.f;
+    DR_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    prof_ms += ms;
+    prof_n += n;
+  }
+};
+
+
+}  // namespace drh
+
+// ---------------------------------------------------------------- C ABI
+struct dr_chain {
+  std::unique_ptr<drh::ChainBase> impl;
+};
+
+using drh::Error;
+using drh::guarded;
+
+extern "C" {
+
+const char* dr_last_error(void) { return drh::g_last_error.c_str(); }
+int dr_version(void) { return 10000; }
+
+int dr_device_count(int* n) {
+  return guarded([&] { DR_HIP(hipGetDeviceCount(n)); });
+}
+
+int dr_create(const dr_config* cfg, const int32_t* ijk, int64_t n_h, dr_chain** out) {
+  return guarded([&] {
+    DR_REQUIRE(cfg && ijk && out && n_h > 0, DR_ELOGIC, "dr_create: null argument or empty data");
+    DR_REQUIRE(cfg->n_datasets == 1 || cfg->n_datasets == 2, DR_ELOGIC,
+               "Gradient only implemented for single or dual resolution");
+    DR_REQUIRE(cfg->precision == DR_FP64 || cfg->precision == DR_FP32, DR_EDOMAIN, "unknown precision");
+    DR_REQUIRE(cfg->covfun == DR_COV_RBF || cfg->covfun == DR_COV_RQ, DR_EDOMAIN, "Unknown covariance function");
+    if (cfg->covfun == DR_COV_RQ) {
+      DR_REQUIRE(cfg->theta[1] > 0, DR_EDOMAIN, "rational_quadratic: psi <= 0");
+      DR_REQUIRE(cfg->theta[2] > 0, DR_EDOMAIN, "rational_quadratic: nu <= 0");
+      DR_REQUIRE(cfg->theta[0] > 0, DR_EDOMAIN, "rational_quadratic: variance <= 0");
+    }
+    std::unique_ptr<drh::ChainBase> c;
+    if (cfg->precision == DR_FP64) c.reset(new drh::Chain<double>());
+    else c.reset(new drh::Chain<float>());
+    c->cfg = *cfg;
+    c->n_h = n_h;
+    c->ijk.assign(ijk, ijk + 3 * n_h);
+    int mx[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n_h; ++i)
+      for (int a = 0; a < 3; ++a) {
+        DR_REQUIRE(ijk[3 * i + a] >= 0, DR_EDOMAIN, "negative data index");
+        mx[a] = std::max(mx[a], ijk[3 * i + a]);
+      }
+    const bool rule = cfg->embed[0] == 0 && cfg->embed[1] == 0 && cfg->embed[2] == 0;
+    for (int a = 0; a < 3; ++a) {
+      if (rule) {
+        const int d = 2 * mx[a];
+        DR_REQUIRE(d >= 2, DR_ELOGIC, "circulant_base_3d: Cannot have dimensions <= 1");
+        c->M[a] = (int)std::pow(2.0, std::ceil(std::log((double)d) / std::log(2.0)));
+      } else {
+        c->M[a] = cfg->embed[a];
+        DR_REQUIRE(c->M[a] >= 2 && c->M[a] % 2 == 0 && c->M[a] >= 2 * mx[a], DR_EDOMAIN,
+                   "embedding must be even and >= 2*max_ijk along every axis");
+      }
+      DR_REQUIRE(c->M[a] <= 2048, DR_EDOMAIN, "embedding axis longer than 2048");
+    }
+    c->build();
+    *out = new dr_chain{std::move(c)};
+  });
+}
+
+int dr_destroy(dr_chain* c) {
+  return guarded([&] { delete c; });
+}
+
+int dr_spectrum_report(dr_chain* c, int dims[3], int64_t* N, int64_t* n_nonpositive, double* max_eig) {
+  return guarded([&] {
+    DR_REQUIRE(c, DR_ELOGIC, "null chain");
+    for (int a = 0; a < 3; ++a) dims[a] = c->impl->M[a];
+    *N = c->impl->N;
+    *n_nonpositive = c->impl->n_nonpos;
+    *max_eig = c->impl->max_eig;
+  });
+}
+
+int dr_get_lambda(dr_chain* c, double* out) {
+  return guarded([&] { c->impl->get_lambda(out); });
+}
+
+int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, int64_t n_s, const int64_t* W_rowptr,
+                const int32_t* W_cols, const double* W_vals) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    DR_REQUIRE(n_h == k.n_h, DR_ELOGIC, "dr_set_data: Yh length differs from the number of data indices");
+    k.Yh.assign(Yh, Yh + n_h);
+    if (k.cfg.n_datasets == 2) {
+      DR_REQUIRE(Ys && n_s > 0 && W_rowptr && W_cols && W_vals, DR_ELOGIC, "Insufficient data for parameters");
+      k.n_s = n_s;
+      k.Ys.assign(Ys, Ys + n_s);
+      k.W_rp.assign(W_rowptr, W_rowptr + n_s + 1);
+      DR_REQUIRE(k.W_rp[0] == 0, DR_EDOMAIN, "W row pointer must start at 0");
+      for (int64_t i = 0; i < n_s; ++i) DR_REQUIRE(k.W_rp[i + 1] >= k.W_rp[i], DR_EDOMAIN, "W row pointer decreasing");
+      const int64_t nnz = k.W_rp[n_s];
+      k.W_cols.assign(W_cols, W_cols + nnz);
+      k.W_vals.assign(W_vals, W_vals + nnz);
+    }
+    k.upload_data();
+  });
+}
+
+int dr_add_data_to_mu(dr_chain* c) {
+  return guarded([&] { c->impl->add_data_to_mu(); });
+}
+
+int dr_set_output_indices(dr_chain* c, const int32_t* ijk, int64_t n_out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    int mx[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n_out; ++i)
+      for (int a = 0; a < 3; ++a) {
+        if (ijk[3 * i + a] < 0) throw Error(DR_EDOMAIN, "MultiResParameters:set_output_indices MIN");
+        if (ijk[3 * i + a] > k.M[a]) throw Error(DR_EDOMAIN, "MultiResParameters:set_output_indices MAX");
+        DR_REQUIRE(ijk[3 * i + a] < k.M[a], DR_EDOMAIN, "output index on the torus boundary is not supported");
+        mx[a] = std::max(mx[a], ijk[3 * i + a]);
+      }
+    k.out_ijk.assign(ijk, ijk + 3 * n_out);
+    int r[3];
+    for (int a = 0; a < 3; ++a) r[a] = std::max(k.reg[a], mx[a] + 1);
+    k.set_region(r);
+    const int rr[3] = {k.reg[0], k.reg[1], k.reg[2]};
+    (void)rr;
+  });
+}
+
+int dr_get_mu_output(dr_chain* c, double* out, int64_t n_out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    std::vector<int32_t> g;
+    if (k.out_ijk.empty()) {
+      g.resize(k.n_h);
+      for (int64_t i = 0; i < k.n_h; ++i)
+        g[i] = (int32_t)k.region_index(k.ijk[3 * i], k.ijk[3 * i + 1], k.ijk[3 * i + 2]);
+    } else {
+      g.resize(k.out_ijk.size() / 3);
+      for (size_t i = 0; i < g.size(); ++i)
+        g[i] = (int32_t)k.region_index(k.out_ijk[3 * i], k.out_ijk[3 * i + 1], k.out_ijk[3 * i + 2]);
+    }
+    DR_REQUIRE((int64_t)g.size() == n_out, DR_ELOGIC, "dr_get_mu_output: wrong output length");
+    k.get_mu_region_gather(g, out);
+  });
+}
+
+int dr_get_mu_data(dr_chain* c, double* out, int64_t n_h) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    DR_REQUIRE(n_h == k.n_h, DR_ELOGIC, "dr_get_mu_data: wrong length");
+    std::vector<int32_t> g(k.n_h);
+    for (int64_t i = 0; i < k.n_h; ++i) g[i] = (int32_t)k.region_index(k.ijk[3 * i], k.ijk[3 * i + 1], k.ijk[3 * i + 2]);
+    k.get_mu_region_gather(g, out);
+  });
+}
+
+int dr_set_sigma_sq_inv(dr_chain* c, const double* s) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    for (int i = 0; i < k.cfg.n_datasets; ++i) k.sigma[i] = s[i];
+  });
+}
+int dr_get_sigma_sq_inv(dr_chain* c, double* s) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    for (int i = 0; i < k.cfg.n_datasets; ++i) s[i] = k.sigma[i];
+  });
+}
+
+int dr_update(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res) {
+  return guarded([&] { c->impl->update(eps, L, u_accept, nullptr, nullptr, res); });
+}
+int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept, const double* z_re, const double* z_im,
+                         dr_update_result* res) {
+  return guarded([&] {
+    DR_REQUIRE(z_re && (z_im || !c->impl->compat()), DR_ELOGIC, "dr_update_with_noise: null noise");
+    c->impl->update(eps, L, u_accept, z_re, z_im, res);
+  });
+}
+
+int dr_log_prior(dr_chain* c, double* out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    k.refresh_stats();
+    *out = -0.5 * k.prior_sum / (double)k.N;
+  });
+}
+int dr_log_likelihood(dr_chain* c, double* out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    DR_REQUIRE(k.data_set, DR_ELOGIC, "Insufficient data for parameters");
+    k.refresh_stats();
+    *out = k.log_lik(k.sse_h, k.sse_s);
+  });
+}
+int dr_log_posterior(dr_chain* c, double* out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    DR_REQUIRE(k.data_set, DR_ELOGIC, "Insufficient data for parameters");
+    k.refresh_stats();
+    *out = -0.5 * k.prior_sum / (double)k.N + k.log_lik(k.sse_h, k.sse_s);
+  });
+}
+int dr_sse(dr_chain* c, double* sse_h, double* sse_s) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    k.refresh_stats();
+    *sse_h = k.sse_h;
+    *sse_s = k.sse_s;
+  });
+}
+
+int dr_set_state(dr_chain* c, const double* mu_re, const double* mu_im, const double* p_re, const double* p_im) {
+  return guarded([&] { c->impl->set_state(mu_re, mu_im, p_re, p_im); });
+}
+int dr_get_state(dr_chain* c, double* mu_re, double* mu_im, double* p_re, double* p_im) {
+  return guarded([&] { c->impl->get_state(mu_re, mu_im, p_re, p_im); });
+}
+int dr_gradient(dr_chain* c, double* g_re, double* g_im) {
+  return guarded([&] { c->impl->gradient(g_re, g_im); });
+}
+int dr_leapfrog(dr_chain* c, double eps, int L, dr_update_result* res) {
+  return guarded([&] { c->impl->leapfrog(eps, L, res); });
+}
+
+int dr_profile_enable(dr_chain* c, int enable) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    k.prof = enable != 0;
+    k.prof_ms = 0.0;
+    k.prof_n = 0;
+  });
+}
+int dr_profile_read(dr_chain* c, double* ms_per_grad, int64_t* n_grads, double* ms_fft_passes, double* ms_w) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    *ms_per_grad = k.prof_n ? k.prof_ms / (double)k.prof_n : 0.0;
+    *n_grads = k.prof_n;
+    if (ms_fft_passes) *ms_fft_passes = 0.0;
+    if (ms_w) *ms_w = 0.0;
+  });
+}
+
+}  // extern "C"

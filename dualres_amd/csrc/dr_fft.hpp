@@ -1,0 +1,353 @@
+// Mixed-radix 3D FFT passes for gfx950, staged through LDS.
+//
+// A 3D transform on the x-fastest grid is three 1D passes.  Each pass kernel
+// loads a tile of TB lines (TB a power of two) into LDS, runs a Stockham
+// autosort FFT over the tile (radices 2,3,4,5,7,8, butterflies in registers,
+// one LDS round trip per stage), and writes the tile back.  Loads and stores
+// go through functor objects so the spectral updates of the leapfrog step,
+// the residual of the likelihood and the energy reductions are fused into
+// the passes instead of costing extra sweeps over HBM.
+//
+//  * k_strided : transform along y or z.  The batch index b (contiguous in
+//                memory) maps to consecutive lanes, so global loads/stores
+//                are coalesced; LDS layout [k][b].
+//  * k_r2c_x   : real rows along x -> half spectrum (M0/2+1), via a complex
+//                FFT of length M0/2 on packed even/odd samples.
+//  * k_c2r_x   : the inverse of k_r2c_x.
+//
+// Transforms are unnormalised (FFTW conventions): forward uses
+// exp(-2 pi i nk/M), inverse exp(+2 pi i nk/M).  Input lengths may be
+// shorter than M (zero padding) and outputs may be truncated, which prunes
+// the passes to the image region of the embedding torus.
+#pragma once
+
+#include <vector>
+
+#include "dr_common.hpp"
+
+namespace drh {
+
+constexpr int kMaxStages = 16;
+constexpr int kTileElems = 4096;  // TB * M per workgroup (16 complex per lane)
+
+template <typename T>
+struct AxisFFT {
+  int M;                  // transform length
+  int nst;                // number of stages
+  int radix[kMaxStages];  // radix of each stage
+  const cx<T>* tw;        // table exp(-2 pi i e / (M * tws)), length M * tws
+  int tws;                // stride into the table
+};
+
+// ---------------------------------------------------------------- butterflies
+template <typename T, int R, bool INV> struct Dft;
+
+template <typename T, bool INV> struct Dft<T, 2, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    const cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 4, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    const cx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+    const cx<T> t2 = v[1] + v[3], t3 = rot90<T, INV>(v[1] - v[3]);
+    v[0] = t0 + t2;
+    v[2] = t0 - t2;
+    v[1] = t1 + t3;
+    v[3] = t1 - t3;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 8, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<T, 4, INV>::run(e);
+    Dft<T, 4, INV>::run(o);
+    const T h = (T)0.70710678118654752440;
+    // w8^k for k = 1,2,3 (conjugated for the inverse)
+    const cx<T> o1 = INV ? mk<T>((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h)
+                         : mk<T>((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    const cx<T> o2 = rot90<T, INV>(o[2]);
+    const cx<T> o3 = INV ? mk<T>(-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h)
+                         : mk<T>((o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h);
+    v[0] = e[0] + o[0];
+    v[4] = e[0] - o[0];
+    v[1] = e[1] + o1;
+    v[5] = e[1] - o1;
+    v[2] = e[2] + o2;
+    v[6] = e[2] - o2;
+    v[3] = e[3] + o3;
+    v[7] = e[3] - o3;
+  }
+};
+
+// Odd radix R = 2m+1 via the symmetric/antisymmetric split:
+//   X_q = x0 + sum_k cos(2 pi kq/R) t_k  -/+ i sum_k sin(2 pi kq/R) u_k
+// with t_k = x_k + x_{R-k}, u_k = x_k - x_{R-k}.
+template <int R> struct OddTab;
+template <> struct OddTab<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <> struct OddTab<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                  -0.80901699437494742410, 0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                  -0.58778525229247312917, -0.95105651629515357212};
+};
+template <> struct OddTab<7> {
+  static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                  -0.90096886790241912624, -0.90096886790241912624,
+                                  -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                  0.43388373911755812048, -0.43388373911755812048,
+                                  -0.97492791218182360702, -0.78183148246802980871};
+};
+
+template <typename T, int R, bool INV> struct DftOdd {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    constexpr int m = R / 2;
+    cx<T> t[m + 1], u[m + 1];
+#pragma unroll
+    for (int k = 1; k <= m; ++k) {
+      t[k] = v[k] + v[R - k];
+      u[k] = v[k] - v[R - k];
+    }
+    const cx<T> x0 = v[0];
+    cx<T> s0 = x0;
+#pragma unroll
+    for (int k = 1; k <= m; ++k) s0 = s0 + t[k];
+#pragma unroll
+    for (int q = 1; q <= m; ++q) {
+      cx<T> a = x0, b = mk<T>(0, 0);
+#pragma unroll
+      for (int k = 1; k <= m; ++k) {
+        const int j = (k * q) % R;
+        a = a + t[k] * (T)OddTab<R>::c[j];
+        b = b + u[k] * (T)OddTab<R>::s[j];
+      }
+      // forward: X_q = a - i b, X_{R-q} = a + i b
+      const cx<T> ib = mk<T>(-b.y, b.x);
+      v[q] = INV ? a + ib : a - ib;
+      v[R - q] = INV ? a - ib : a + ib;
+    }
+    v[0] = s0;
+  }
+};
+template <typename T, bool INV> struct Dft<T, 3, INV> : DftOdd<T, 3, INV> {};
+template <typename T, bool INV> struct Dft<T, 5, INV> : DftOdd<T, 5, INV> {};
+template <typename T, bool INV> struct Dft<T, 7, INV> : DftOdd<T, 7, INV> {};
+
+// ---------------------------------------------------------------- Stockham stage
+// LDS holds TB = 2^tbs lines; element k of line b sits at lds[k*P + b].
+// Stage with span Ns: butterfly j reads j + r*M/R, twiddles by
+// w^(r * (j mod Ns) * M/(Ns R)), writes (j - j mod Ns) R + j mod Ns + r Ns.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void fft_stage(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax, int Ns) {
+  const int TB = 1 << tbs;
+  const int BF = ax.M / R;
+  const int total = BF << tbs;
+  const int estep = (ax.M / (Ns * R)) * ax.tws;
+  constexpr int IT = (16 + R - 1) / R;
+  cx<T> v[IT][R];
+  int dst[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int q = threadIdx.x + it * kThreads;
+    dst[it] = -1;
+    if (q < total) {
+      const int b = q & (TB - 1);
+      const int j = q >> tbs;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = lds[(j + r * BF) * P + b];
+      const int jm = j % Ns;
+      if (jm != 0) {
+        const int e = jm * estep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cx<T> w = ax.tw[r * e];
+          v[it][r] = INV ? cmulc(v[it][r], w) : cmul(v[it][r], w);
+        }
+      }
+      Dft<T, R, INV>::run(v[it]);
+      dst[it] = ((j - jm) * R + jm) * P + b;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (dst[it] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dst[it] + r * Ns * P] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax) {
+  int Ns = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    const int R = ax.radix[s];
+    switch (R) {
+      case 8: fft_stage<T, 8, INV>(lds, P, tbs, ax, Ns); break;
+      case 4: fft_stage<T, 4, INV>(lds, P, tbs, ax, Ns); break;
+      case 2: fft_stage<T, 2, INV>(lds, P, tbs, ax, Ns); break;
+      case 3: fft_stage<T, 3, INV>(lds, P, tbs, ax, Ns); break;
+      case 5: fft_stage<T, 5, INV>(lds, P, tbs, ax, Ns); break;
+      case 7: fft_stage<T, 7, INV>(lds, P, tbs, ax, Ns); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------- pass kernels
+// Functor interfaces:
+//   strided:  cx<T> Pro::load(int o, int k, int b);   void Epi::store(int o, int k, int b, cx<T>);
+//   r2c_x:    cx<T> Pro::load2(int row, int n);  (samples 2n, 2n+1 packed)   void Epi::store(row, k, cx<T>)
+//   c2r_x:    cx<T> Pro::load(int row, int k);   void Epi::store2(int row, int n, cx<T> z)  (samples 2n, 2n+1)
+// and void finish(double* red, int block) on both for optional reductions.
+
+template <typename T, bool INV, class Pro, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[kThreads];
+  const int TB = 1 << tbs;
+  const int o = blockIdx.y;
+  const int b0 = blockIdx.x * TB;
+  const int M = ax.M;
+  for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    cx<T> v = mk<T>(0, 0);
+    if (k < k_in && b0 + b < NB) v = pro.load(o, k, b0 + b);
+    lds[k * TB + b] = v;
+  }
+  __syncthreads();
+  fft_lds<T, INV>(lds, TB, tbs, ax);
+  for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    if (b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b]);
+  }
+  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+  pro.finish(red, blk);
+  epi.finish(red, blk);
+}
+
+// Real rows of length M0 (first n_in samples nonzero) -> M0/2+1 complex.
+// ax describes the complex FFT of length Mh = M0/2 with tws = 2 into the
+// length-M0 table, so ax.tw[k] = exp(-2 pi i k / M0).
+template <typename T, class Pro, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[kThreads];
+  const int TB = 1 << tbs;
+  const int P = TB + 1;
+  const int Mh = ax.M;
+  const int row0 = blockIdx.x * TB;
+  for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
+    const int l = idx / Mh, n = idx - l * Mh;
+    const int row = row0 + l;
+    cx<T> v = mk<T>(0, 0);
+    if (row < nrows) v = pro.load2(row, n);
+    lds[n * P + l] = v;
+  }
+  __syncthreads();
+  fft_lds<T, false>(lds, P, tbs, ax);
+  const int H = Mh + 1;
+  for (int idx = threadIdx.x; idx < TB * H; idx += kThreads) {
+    const int l = idx / H, k = idx - l * H;
+    const int row = row0 + l;
+    if (row < nrows) {
+      const cx<T> zk = lds[(k == Mh ? 0 : k) * P + l];
+      const cx<T> zc = conj(lds[(k == 0 ? 0 : Mh - k) * P + l]);
+      const cx<T> e = (zk + zc) * (T)0.5;
+      const cx<T> d = zk - zc;
+      const cx<T> od = mk<T>(d.y * (T)0.5, -d.x * (T)0.5);  // (zk - zc) / (2i)
+      const cx<T> w = ax.tw[k];                              // exp(-2 pi i k / M0)
+      epi.store(row, k, e + cmul(w, od));
+    }
+  }
+  const int blk = blockIdx.x;
+  pro.finish(red, blk);
+  epi.finish(red, blk);
+}
+
+// M0/2+1 complex per row -> real rows; the epilogue receives packed pairs
+// (x[2n], x[2n+1]) = (Re z[n], Im z[n]) unscaled.
+template <typename T, class Pro, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[kThreads];
+  const int TB = 1 << tbs;
+  const int P = TB + 1;
+  const int Mh = ax.M;
+  const int H = Mh + 1;
+  const int row0 = blockIdx.x * TB;
+  for (int idx = threadIdx.x; idx < TB * H; idx += kThreads) {
+    const int l = idx / H, k = idx - l * H;
+    const int row = row0 + l;
+    cx<T> v = mk<T>(0, 0);
+    if (row < nrows) v = pro.load(row, k);
+    lds[k * P + l] = v;
+  }
+  __syncthreads();
+  // Z[k] = (X[k] + conj X[Mh-k]) + i (X[k] - conj X[Mh-k]) exp(+2 pi i k / M0)
+  const int hp = Mh / 2 + 1;
+  for (int idx = threadIdx.x; idx < TB * hp; idx += kThreads) {
+    const int l = idx / hp, k = idx - l * hp;
+    const cx<T> a = lds[k * P + l];
+    const cx<T> c = lds[(Mh - k) * P + l];
+    const cx<T> s1 = a + conj(c), d1 = a - conj(c);
+    const cx<T> t1 = cmulc(d1, ax.tw[k]);
+    const cx<T> zk = mk<T>(s1.x - t1.y, s1.y + t1.x);
+    if (k > 0 && Mh - k != k) {
+      const cx<T> s2 = c + conj(a), d2 = c - conj(a);
+      const cx<T> t2 = cmulc(d2, ax.tw[Mh - k]);
+      lds[(Mh - k) * P + l] = mk<T>(s2.x - t2.y, s2.y + t2.x);
+    }
+    lds[k * P + l] = zk;
+  }
+  __syncthreads();
+  fft_lds<T, true>(lds, P, tbs, ax);
+  for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
+    const int l = idx / Mh, n = idx - l * Mh;
+    const int row = row0 + l;
+    if (row < nrows) epi.store2(row, n, lds[n * P + l]);
+  }
+  const int blk = blockIdx.x;
+  pro.finish(red, blk);
+  epi.finish(red, blk);
+}
+
+// ---------------------------------------------------------------- host plan helpers
+std::vector<int> factorize(int M);
+
+template <typename T>
+AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws) {
+  AxisFFT<T> a{};
+  a.M = fft_len;
+  std::vector<int> f = factorize(fft_len);
+  if ((int)f.size() > kMaxStages) throw Error(DR_EDOMAIN, "FFT length needs too many stages");
+  a.nst = (int)f.size();
+  for (int i = 0; i < a.nst; ++i) a.radix[i] = f[i];
+  a.tw = tw;
+  a.tws = tws;
+  return a;
+}
+
+// Largest power-of-two tile TB with TB*len <= tile_elems and the LDS bytes
+// within 64 KiB; x-passes use a padded pitch TB+1 and len+1 rows.
+int pick_tbs(int len, int rows_lds, int elem_bytes, bool padded, int max_lines, int tile_elems);
+
+}  // namespace drh

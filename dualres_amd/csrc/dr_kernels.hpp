@@ -1,0 +1,345 @@
+// Load/store functors fused into the FFT passes, and the non-FFT kernels
+// of the gradient: CSR W.mu, W^T v, reductions.
+//
+// Spectral arrays hold the half spectrum S[kz][ky][kx], kx in [0, M0/2],
+// of real torus fields.  invlam[k] = 1/lambda_k for lambda_k > 0 and 0 on
+// the low-rank set Z (Re lambda <= 0), which implements every "zero Z" of
+// the reference (MultiResParameters.h:309-319) with a single array.
+#pragma once
+
+#include "dr_fft.hpp"
+
+namespace drh {
+
+struct NoRed {
+  __device__ __forceinline__ void finish(double*, int) {}
+};
+
+// weight of a half-spectrum entry in a full-torus sum: 1 on the kx = 0 and
+// kx = M0/2 planes, 2 elsewhere (Hermitian partner not stored)
+__device__ __forceinline__ double herm_w(int kx, int H0) { return (kx == 0 || kx == H0 - 1) ? 1.0 : 2.0; }
+
+__device__ __forceinline__ void red_write(double v, double* red, double* part, int blk) {
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[blk] = s;
+}
+
+// ---------------------------------------------------------------- strided functors
+template <typename T> struct LoadS : NoRed {
+  const cx<T>* src; long so, sk;
+  __device__ __forceinline__ cx<T> load(int o, int k, int b) const { return src[o * so + (long)k * sk + b]; }
+};
+template <typename T> struct StoreS : NoRed {
+  cx<T>* dst; long so, sk;
+  __device__ __forceinline__ void store(int o, int k, int b, cx<T> v) const { dst[o * so + (long)k * sk + b] = v; }
+};
+
+// position update mu += eps * M^{-1} p fused into the first inverse pass
+// (MultiResParameters.h:693-698): M^{-1} = 1 / (1/lambda + sigma_h^-2), 0 on Z.
+template <typename T> struct LoadPosUpdate : NoRed {
+  const cx<T>* src; cx<T>* dst; const cx<T>* p; const T* invlam; long sk; T eps; T sig;
+  const cx<T>* srci; cx<T>* dsti; const cx<T>* pim;  // imaginary chain (may be null)
+  __device__ __forceinline__ cx<T> load(int, int k, int b) const {
+    const long i = (long)k * sk + b;
+    const T il = invlam[i];
+    const T c = il > (T)0 ? eps / (il + sig) : (T)0;
+    const cx<T> m = src[i] + p[i] * c;
+    dst[i] = m;
+    if (dsti) dsti[i] = srci[i] + pim[i] * c;
+    return m;
+  }
+};
+
+// momentum update p += coef * (F(S r) - mu/lambda) fused into the last
+// forward pass (MultiResParameters.h:690,699 with _compute_gradient :349-374);
+// optional reductions of the prior and kinetic energies (Parseval).
+template <typename T> struct EpiMomentum {
+  const cx<T>* X; cx<T>* p; const T* invlam; long sk; T coef; T sig; int H0;
+  const cx<T>* Xi; cx<T>* pim;
+  int do_red; double* part; long pstride; int s_prior, s_priori, s_kin, s_kini;
+  double a0, a1, a2, a3;
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) {
+    const long i = (long)k * sk + b;
+    const T il = invlam[i];
+    const cx<T> x = X[i];
+    const cx<T> pn = p[i] + (v - x * il) * coef;
+    p[i] = pn;
+    cx<T> xi = mk<T>(0, 0), pin = mk<T>(0, 0);
+    if (pim) {
+      xi = Xi[i];
+      pin = pim[i] - xi * (il * coef);
+      pim[i] = pin;
+    }
+    if (do_red) {
+      const double w = herm_w(b % H0, H0);
+      const double ild = (double)il;
+      const double ilm = il > (T)0 ? 1.0 / (double)(il + sig) : 0.0;
+      a0 += w * ild * ((double)x.x * x.x + (double)x.y * x.y);
+      a2 += w * ilm * ((double)pn.x * pn.x + (double)pn.y * pn.y);
+      a1 += w * ild * ((double)xi.x * xi.x + (double)xi.y * xi.y);
+      a3 += w * ilm * ((double)pin.x * pin.x + (double)pin.y * pin.y);
+    }
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (!do_red) return;
+    red_write(a0, red, part + s_prior * pstride, blk);
+    red_write(a1, red, part + s_priori * pstride, blk);
+    red_write(a2, red, part + s_kin * pstride, blk);
+    red_write(a3, red, part + s_kini * pstride, blk);
+  }
+};
+
+// gradient in the spectral domain: G = F(S r) - X/lambda
+template <typename T> struct EpiGrad : NoRed {
+  const cx<T>* X; cx<T>* G; const T* invlam; long sk;
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
+    const long i = (long)k * sk + b;
+    G[i] = v - X[i] * invlam[i];
+  }
+};
+
+// momentum draw p = sqrt(lambda_m) F(z), 0 on Z (MultiResParameters.h:811-814)
+template <typename T> struct EpiNoiseMomentum : NoRed {
+  cx<T>* p; const T* invlam; long sk; T sig;
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
+    const long i = (long)k * sk + b;
+    const T il = invlam[i];
+    p[i] = il > (T)0 ? v * (T)sqrt((double)(il + sig)) : mk<T>(0, 0);
+  }
+};
+
+// initial state mu = sqrt(lambda) F(z) / 10, 0 on Z (MultiResParameters.h:752-755)
+template <typename T> struct EpiInitMu : NoRed {
+  cx<T>* X; const double* lam; long sk;
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
+    const long i = (long)k * sk + b;
+    const double l = lam[i];
+    X[i] = l > 0.0 ? v * (T)(sqrt(l) / 10.0) : mk<T>(0, 0);
+  }
+};
+
+template <typename T> struct EpiAddTo : NoRed {
+  cx<T>* X; long sk;
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
+    const long i = (long)k * sk + b;
+    X[i] = X[i] + v;
+  }
+};
+
+// eigenvalues: lambda = Re F(c); invlam for the working precision; weighted
+// count of Re lambda <= 0 (MultiResParameters.h:253-262)
+template <typename T> struct EpiSpectrum {
+  double* lam; T* invlam; long sk; int H0; double* part; double acc;
+  __device__ __forceinline__ void store(int, int k, int b, cx<double> v) {
+    const long i = (long)k * sk + b;
+    lam[i] = v.x;
+    invlam[i] = v.x > 0.0 ? (T)(1.0 / v.x) : (T)0;
+    if (!(v.x > 0.0)) acc += herm_w(b % H0, H0);
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) { red_write(acc, red, part, blk); }
+};
+
+// ---------------------------------------------------------------- x-pass functors
+template <typename T> struct LoadRowC : NoRed {
+  const cx<T>* src; int H0;
+  __device__ __forceinline__ cx<T> load(int row, int k) const { return src[(long)row * H0 + k]; }
+};
+template <typename T> struct StoreRowC : NoRed {
+  cx<T>* dst; int H0;
+  __device__ __forceinline__ void store(int row, int k, cx<T> v) const { dst[(long)row * H0 + k] = v; }
+};
+template <typename T> struct LoadRowR : NoRed {
+  const T* src; int n_in;
+  __device__ __forceinline__ cx<T> load2(int row, int n) const {
+    const int x = 2 * n;
+    const long base = (long)row * n_in;
+    return mk<T>(x < n_in ? src[base + x] : (T)0, x + 1 < n_in ? src[base + x + 1] : (T)0);
+  }
+};
+template <typename T> struct StoreRowR : NoRed {
+  T* dst; int n_out; T scale;
+  __device__ __forceinline__ void store2(int row, int n, cx<T> z) const {
+    const int x = 2 * n;
+    const long base = (long)row * n_out;
+    if (x < n_out) dst[base + x] = z.x * scale;
+    if (x + 1 < n_out) dst[base + x + 1] = z.y * scale;
+  }
+};
+
+// standard normal field on the full torus, pair index = flat / 2; the float64
+// sum of squares of the working-precision values gives E0 (:785-809)
+template <typename T> struct LoadNoise {
+  uint64_t seed, stream; int M0h; double* part; double acc;
+  __device__ __forceinline__ cx<T> load2(int row, int n) {
+    double z0, z1;
+    normal_pair(seed, stream, (uint64_t)row * (uint64_t)M0h + (uint64_t)n, z0, z1);
+    const T a = (T)z0, b = (T)z1;
+    acc += (double)a * (double)a + (double)b * (double)b;
+    return mk<T>(a, b);
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) { red_write(acc, red, part, blk); }
+};
+// injected normal field (test hook), same E0 reduction
+template <typename T> struct LoadNoiseBuf {
+  const T* z; int M0; double* part; double acc;
+  __device__ __forceinline__ cx<T> load2(int row, int n) {
+    const long i = (long)row * M0 + 2 * n;
+    const T a = z[i], b = z[i + 1];
+    acc += (double)a * (double)a + (double)b * (double)b;
+    return mk<T>(a, b);
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) { red_write(acc, red, part, blk); }
+};
+
+// likelihood residual on the image region, scattered onto the torus rows
+// (MultiResParameters.h:357-374): r = sigma_h^-2 (Yh - mu) + W^T v at data
+// voxels, 0 elsewhere; accumulates ||Yh - mu[I]||^2 (:502-503).
+template <typename T> struct LoadResidual {
+  const T* R; const T* Yg; const uint8_t* mask; const T* WTv; T sig; int n0; double* part; double acc;
+  __device__ __forceinline__ T one(long g) {
+    if (!mask[g]) return (T)0;
+    const T d = Yg[g] - R[g];
+    acc += (double)d * (double)d;
+    T r = sig * d;
+    if (WTv) r += WTv[g];
+    return r;
+  }
+  __device__ __forceinline__ cx<T> load2(int row, int n) {
+    const int x = 2 * n;
+    const long base = (long)row * n0;
+    const T a = x < n0 ? one(base + x) : (T)0;
+    const T b = x + 1 < n0 ? one(base + x + 1) : (T)0;
+    return mk<T>(a, b);
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (part) red_write(acc, red, part, blk);
+  }
+};
+
+// circulant base c = cov(||Q (d(i), d(j), d(k))||) with d(t) = min(t, M-t);
+// distance in float32 (impl/circulant_base.inl:60-85), kernel in float64
+// (impl/kernels.inl:7-15, 58-74).
+struct LoadBase : NoRed {
+  int M0, M1, M2; float q[9]; double tau2, psi, nu; int covfun;
+  __device__ __forceinline__ double cov(int x, int y, int z) const {
+    const float dx = (float)(x <= M0 / 2 ? x : M0 - x);
+    const float dy = (float)(y <= M1 / 2 ? y : M1 - y);
+    const float dz = (float)(z <= M2 / 2 ? z : M2 - z);
+    float c[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      float v = __fmul_rn(q[3 * r + 0], dx);
+      v = __fadd_rn(v, __fmul_rn(q[3 * r + 1], dy));
+      v = __fadd_rn(v, __fmul_rn(q[3 * r + 2], dz));
+      c[r] = v;
+    }
+    const float ss = __fadd_rn(__fadd_rn(__fmul_rn(c[0], c[0]), __fmul_rn(c[1], c[1])), __fmul_rn(c[2], c[2]));
+    const double d = (double)sqrt_rn(ss);
+    if (covfun == 0) return tau2 * exp(-psi * pow(fabs(d), nu));
+    return tau2 * pow(1.0 + psi / nu * d * d, -nu);
+  }
+  __device__ __forceinline__ cx<double> load2(int row, int n) const {
+    const int y = row % M1, z = row / M1;
+    return mk<double>(cov(2 * n, y, z), cov(2 * n + 1, y, z));
+  }
+};
+
+// ---------------------------------------------------------------- other kernels
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// v_i = sigma_s^-2 (Ys_i - sum_j W_ij mu[col_ij]); one wavefront per row;
+// accumulates ||Ys - W mu||^2 (MultiResParameters.h:361-363, 505-507).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wmu(int n_s, const int* __restrict__ rp, const int* __restrict__ cols, const T* __restrict__ vals,
+      const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, T* __restrict__ v, double* part) {
+  __shared__ double red[kThreads];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  double e2 = 0.0;
+  if (row < n_s) {
+    T acc = (T)0;
+    const int j1 = rp[row + 1];
+    for (int j = rp[row] + lane; j < j1; j += 64) acc += vals[j] * R[cols[j]];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const T e = Ys[row] - acc;
+      v[row] = sig_s * e;
+      e2 = (double)e * (double)e;
+    }
+  }
+  if (part) red_write(e2, red, part, blockIdx.x);
+}
+
+// (W^T v) at each data voxel, written to its image-region grid position;
+// transposed CSR, 16 lanes per data voxel (deterministic order).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wtv(int n_h, const int* __restrict__ rp, const int* __restrict__ rows, const T* __restrict__ vals,
+      const T* __restrict__ v, const int* __restrict__ gidx, T* __restrict__ out) {
+  const int lane = threadIdx.x & 15;
+  const int d = blockIdx.x * (kThreads / 16) + (threadIdx.x >> 4);
+  T acc = (T)0;
+  if (d < n_h) {
+    const int j1 = rp[d + 1];
+    for (int j = rp[d] + lane; j < j1; j += 16) acc += vals[j] * v[rows[j]];
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+  if (d < n_h && lane == 0) out[gidx[d]] = acc;
+}
+
+// sum over the half spectrum of w |X|^2 a(k) with a = invlam (prior) or
+// 1/(invlam + sig) (kinetic); sig < 0 selects the prior form.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_quad(long Nh, int H0, const cx<T>* __restrict__ X, const T* __restrict__ invlam, T sig, double* part) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < Nh; i += (long)gridDim.x * kThreads) {
+    const T il = invlam[i];
+    double a = (double)il;
+    if (sig >= (T)0) a = il > (T)0 ? 1.0 / (double)(il + sig) : 0.0;
+    const cx<T> x = X[i];
+    acc += herm_w((int)(i % H0), H0) * a * ((double)x.x * x.x + (double)x.y * x.y);
+  }
+  red_write(acc, red, part, blockIdx.x);
+}
+
+// ||Yh - mu[I]||^2 over the region
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_sse_h(long n, const T* __restrict__ R, const T* __restrict__ Yg, const uint8_t* __restrict__ mask, double* part) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (long g = (long)blockIdx.x * kThreads + threadIdx.x; g < n; g += (long)gridDim.x * kThreads) {
+    if (mask[g]) {
+      const T d = Yg[g] - R[g];
+      acc += (double)d * (double)d;
+    }
+  }
+  red_write(acc, red, part, blockIdx.x);
+}
+
+struct FinalizeArgs { int nslot; int nblk[32]; long pstride; };
+__global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a, const double* part, double* sums);
+
+template <typename T>
+__global__ void k_gather(long n, const int* __restrict__ idx, const T* __restrict__ src, T* __restrict__ dst) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+template <typename T>
+__global__ void k_scale_spec(long n, cx<T>* X, const T* __restrict__ invlam, T s) {
+  // X = s * X * invlam  (imaginary-chain prior gradient)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) X[i] = X[i] * (s * invlam[i]);
+}
+
+}  // namespace drh

@@ -1,0 +1,184 @@
+/*
+ * dualres_hip.h -- C ABI of the MI355X (gfx950) engine for the HMC posterior
+ * sampler of the dual-resolution Gaussian-process image model.
+ *
+ * Every entry point returns a dr_status; on failure a message is available
+ * from dr_last_error() (thread-local).  No C++ exception crosses this ABI.
+ * Inputs are borrowed host pointers copied during the call; outputs go to
+ * caller-allocated host buffers.  A dr_chain is single-threaded: calls on
+ * one handle must not overlap.  Citations name the reference function each
+ * entry point replaces (paths relative to the asw221/dualres checkout).
+ */
+#ifndef DUALRES_HIP_H_
+#define DUALRES_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; mirror the std:: exception kinds the reference throws. */
+enum dr_status {
+  DR_OK = 0,
+  DR_EDOMAIN = 1,  /* std::domain_error  (e.g. MultiResParameters.h:283-284) */
+  DR_ELOGIC = 2,   /* std::logic_error   (e.g. MultiResParameters.h:366-367) */
+  DR_ERUNTIME = 3, /* std::runtime_error                                     */
+  DR_EHIP = 4,     /* HIP runtime failure                                    */
+  DR_ENOMEM = 5,   /* device or host allocation failed                       */
+  DR_EBUSY = 6     /* serving: admission refused / no slot                   */
+};
+
+enum dr_precision {
+  DR_FP64 = 0,     /* gradient, FFTs and state in float64 (dualgpm)          */
+  DR_FP32 = 1      /* FFTs/state/gradient in float32, eigenvalues, energies,
+                      accept ratio and kriging solves in float64            */
+};
+
+enum dr_state_mode {
+  DR_STATE_COMPAT = 0, /* also evolve the data-free imaginary chain so that the
+                          energies and accept ratio equal the reference's
+                          complex-state ones (MultiResParameters.h:428-476,
+                          551-603, 783-815)                                 */
+  DR_STATE_REAL = 1    /* real state only (the imaginary chain is dropped)  */
+};
+
+enum dr_covfun { DR_COV_RBF = 0, DR_COV_RQ = 1 };
+
+/* Chain configuration.  Replaces the MultiResParameters<T> constructor
+ * arguments (MultiResParameters.h:45-53, :172-300). */
+typedef struct dr_config {
+  int n_datasets;      /* 1 (highres only) or 2 (highres + stdres)          */
+  int precision;       /* dr_precision                                      */
+  int state_mode;      /* dr_state_mode                                     */
+  int covfun;          /* dr_covfun                                         */
+  int embed[3];        /* embedding torus (M0,M1,M2); all 0 = reference rule
+                          2^ceil(log2(2*max_ijk)) (MultiResParameters.h:191-207);
+                          otherwise each must be even and >= 2*max_ijk      */
+  double theta[3];     /* covariance parameters (variance tau^2, bandwidth
+                          psi, exponent nu), reference order                */
+  float qform[16];     /* highres qto_xyz, row-major 4x4 (nifti_manipulation.inl:543-551) */
+  uint64_t seed;       /* Philox key for device normals                     */
+  int device;          /* HIP device ordinal                                */
+} dr_config;
+
+typedef struct dr_chain dr_chain;
+
+/* Result of one HMC iteration (MultiResParameters.h:673-727). */
+typedef struct dr_update_result {
+  double mh_rate;      /* min(1, R), the value update() returns            */
+  double R;            /* exp(H1 - H0), NaN mapped to 0                    */
+  double H0, H1;       /* initial / proposed total energies                */
+  double log_prior0, log_lik0, e0;  /* pieces of H0                         */
+  double log_prior1, log_lik1, k1;  /* pieces of H1                         */
+  double sse_h, sse_s; /* ||Yh - mu[I]||^2, ||Ys - W mu[I]||^2 of the state
+                          kept after the accept step (Gibbs rates,
+                          MultiResParameters.h:826-837)                     */
+  int accepted;
+} dr_update_result;
+
+const char* dr_last_error(void);
+int dr_version(void);
+int dr_device_count(int* n);
+
+/* Constructor: spectrum, low-rank set, initial mu (a GP draw / 10).
+ * ijk: n_h x 3 row-major data indices relative to the input-mask bounding
+ * box (fit_model, gaussian_process_model.h:107-113).
+ * Errors: DR_EDOMAIN when >= N/2 eigenvalues are nonpositive
+ * (MultiResParameters.h:283-284) or the embedding is invalid. */
+int dr_create(const dr_config* cfg, const int32_t* ijk, int64_t n_h, dr_chain** out);
+int dr_destroy(dr_chain* c);
+
+/* Spectrum report (MultiResParameters.h:258-281): embedding dims, N, the
+ * number of eigenvalues with real part <= 0 and lambda[0]/N. */
+int dr_spectrum_report(dr_chain* c, int dims[3], int64_t* N, int64_t* n_nonpositive, double* max_eig);
+/* Real parts of lambda on the full torus (length N, x fastest). */
+int dr_get_lambda(dr_chain* c, double* lambda_out);
+
+/* Data (MultiResData.h:96-127): Yh in nonzero order (length n_h), Ys
+ * (length n_s) and W as row-major CSR with columns indexing Yh.  For
+ * n_datasets == 1 pass n_s = 0 and NULL W arrays. */
+int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, int64_t n_s,
+                const int64_t* W_rowptr, const int32_t* W_cols, const double* W_vals);
+
+/* MultiResParameters::add_data_to_mu (:631-641). */
+int dr_add_data_to_mu(dr_chain* c);
+/* MultiResParameters::set_output_indices (:646-667). */
+int dr_set_output_indices(dr_chain* c, const int32_t* ijk, int64_t n_out);
+/* MultiResParameters::mu_ (:402-408): Re(mu) at the output indices. */
+int dr_get_mu_output(dr_chain* c, double* out, int64_t n_out);
+/* MultiResParameters::mu (:393-397): Re(mu) at the data indices. */
+int dr_get_mu_data(dr_chain* c, double* out, int64_t n_h);
+
+/* Noise precisions sigma^-2 (length n_datasets).  The reference draws them
+ * in _initialize_sigma (:764-778) and _update_sigma_sq_inv (:820-859) from
+ * its host RNG; here the host passes the drawn values in. */
+int dr_set_sigma_sq_inv(dr_chain* c, const double* s);
+int dr_get_sigma_sq_inv(dr_chain* c, double* s);
+
+/* One HMC iteration (MultiResParameters::update, :673-727) with momentum
+ * normals from the device Philox stream and the host-drawn accept uniform
+ * u_accept in [0,1).  Does not perform the Gibbs draws. */
+int dr_update(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res);
+/* Same with injected momentum normals z = z_re + i z_im on the full torus
+ * (length N each, x fastest); z_im is ignored in DR_STATE_REAL. */
+int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept,
+                         const double* z_re, const double* z_im, dr_update_result* res);
+
+/* Energies of the current state: log_prior (:428-476), log_likelihood
+ * (:488-514), log_posterior (:528-544). */
+int dr_log_prior(dr_chain* c, double* out);
+int dr_log_likelihood(dr_chain* c, double* out);
+int dr_log_posterior(dr_chain* c, double* out);
+/* Current ||Yh - mu[I]||^2 and ||Ys - W mu[I]||^2. */
+int dr_sse(dr_chain* c, double* sse_h, double* sse_s);
+
+/* --- deterministic test hooks ------------------------------------------ */
+/* Full-torus state in real space (length N each, x fastest). */
+int dr_set_state(dr_chain* c, const double* mu_re, const double* mu_im,
+                 const double* p_re, const double* p_im);
+int dr_get_state(dr_chain* c, double* mu_re, double* mu_im, double* p_re, double* p_im);
+/* Gradient at the current mu (MultiResParameters::gradient, :382-389),
+ * full torus, real and imaginary parts. */
+int dr_gradient(dr_chain* c, double* g_re, double* g_im);
+/* L leapfrog steps from the current (mu, p) with the mass matrix of the
+ * current sigma (MultiResParameters.h:690-700); the end point becomes the
+ * current state.  Energies before/after are returned in res (e0 and k1 are
+ * the kinetic terms of the initial and final momentum). */
+int dr_leapfrog(dr_chain* c, double eps, int L, dr_update_result* res);
+
+/* Timing: average device time (ms) of the last n_events leapfrog steps
+ * recorded with HIP events on the chain's stream (0 if none). */
+int dr_profile_enable(dr_chain* c, int enable);
+int dr_profile_read(dr_chain* c, double* ms_per_grad, int64_t* n_grads,
+                    double* ms_fft_passes, double* ms_w);
+
+/* --- kriging matrix build (kriging_matrix.h:147-343) --------------------- */
+typedef struct dr_csr dr_csr;
+/* hr/std: dense float64 volumes (nx*ny*nz, x fastest), 0 or NaN = missing,
+ * already masked.  theta as in dr_config.  Pattern is the reference's
+ * (neighbour order, flat-offset range check); weights are float64 solves
+ * normalised by their sum (rows with a NaN sum are left empty). */
+int dr_kriging_build(const double* hr, const int hr_dims[3], const float q_hr[16],
+                     const double* std_img, const int std_dims[3], const float q_std[16],
+                     const double theta[3], double radius, int device, dr_csr** out);
+int dr_csr_info(dr_csr* m, int64_t* nrow, int64_t* ncol, int64_t* nnz, int64_t* n_unique_solves);
+int dr_csr_copy(dr_csr* m, int64_t* rowptr, int32_t* cols, double* vals);
+int dr_csr_free(dr_csr* m);
+
+/* --- minimum contrast summary (estimate_kernel_parameters.h:53-101) ------- */
+/* Number of MCE offsets (perturbation_matrix.h:62-106; 25,348). */
+int dr_mce_offsets(int64_t* n, int32_t* offsets_xyz /* may be NULL; else n*3 */);
+/* img: dense float64 volume (x fastest), 0/NaN missing. Outputs per offset:
+ * covariance (float), distance (float), npairs (int). */
+int dr_mce_summary(const double* img, const int dims[3], const float qform[16], int device,
+                   float* covariance, float* distance, int32_t* npairs);
+/* Binned empirical covariance: nbins distance bins on [0, maxdist] over
+ * all offsets with |offset| <= maxdist (FP64 sums per bin). */
+int dr_mce_binned(const double* img, const int dims[3], const float qform[16], int device,
+                  int nbins, double maxdist, double* bin_cov, double* bin_dist, int64_t* bin_npairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DUALRES_HIP_H_ */

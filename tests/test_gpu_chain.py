@@ -1,0 +1,171 @@
+"""GPU parity of the HMC chain against the CPU oracle (float64 and float32).
+
+The oracle (oracle/mrp.py) restates MultiResParameters.h with complex state
+and c2c FFTs; the engine runs a Fourier-domain real state with pruned
+r2c/c2r passes.  Both are driven from the same injected state and noise.
+
+Tolerances (relative 2-norm against the float64 oracle):
+  float64 engine: eigenvalues 1e-12 of max|lambda|, gradient 1e-10,
+                  leapfrog trajectory 1e-9, energies 1e-9 (relative);
+  float32 engine: gradient 2e-4, trajectory 2e-3, energies 1e-4.
+"""
+import numpy as np
+import pytest
+
+from helpers import engine_chain, oracle_chain, random_state, rel, small_dual, small_single
+from oracle import mrp
+
+pytestmark = pytest.mark.gpu
+
+THETA1 = (0.806, 0.132, 1.0)
+THETA2 = (0.806, 0.132, 1.5)
+
+
+def _setup(p, theta, precision, mode, embed=None, sig=(1.3, 0.7), seed=0):
+    o, data = oracle_chain(p, theta, embed=embed)
+    c = engine_chain(p, theta, precision=precision, state_mode=mode, embed=embed)
+    rng = np.random.default_rng(seed)
+    mu, mom = random_state(o, rng, imag=(mode == "compat"))
+    mu.reshape(-1)[o.I] += p["Yh"]
+    o.mu = mu.copy()
+    o.sigma_sq_inv = [np.float64(s) for s in sig[: o.n_datasets]]
+    c.set_state(mu, mom)
+    c.set_sigma_sq_inv(list(sig[: o.n_datasets]))
+    return o, data, c, mu, mom
+
+
+@pytest.mark.parametrize("embed", [None, (20, 24, 18)])
+def test_eigenvalues_fp64(gpu_available, embed):
+    p = small_single()
+    o, _ = oracle_chain(p, THETA1, embed=embed)
+    c = engine_chain(p, THETA1, embed=embed)
+    assert c.M == o.M
+    lam_e = c.lambda_()
+    lam_o = o.lam.real
+    assert np.max(np.abs(lam_e - lam_o)) <= 1e-12 * np.max(np.abs(lam_o))
+    assert c.n_nonpositive == o.Z.size
+    assert abs(c.max_eig - lam_o.ravel()[0] / o.N) <= 1e-12 * abs(lam_o.ravel()[0] / o.N)
+
+
+@pytest.mark.parametrize("embed", [None, (20, 24, 18)])
+def test_state_roundtrip(gpu_available, embed):
+    p = small_single()
+    c = engine_chain(p, THETA1, embed=embed)
+    rng = np.random.default_rng(1)
+    mu = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    pm = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    c.set_state(mu, pm)
+    mu2, pm2 = c.get_state()
+    assert rel(mu2, mu) < 1e-13
+    assert rel(pm2, pm) < 1e-13
+
+
+@pytest.mark.parametrize("mode", ["compat", "real"])
+@pytest.mark.parametrize("embed", [None, (20, 24, 18)])
+def test_gradient_single_fp64(gpu_available, mode, embed):
+    p = small_single()
+    o, data, c, mu, _ = _setup(p, THETA1, "fp64", mode, embed)
+    g_o = o.compute_gradient(data, o.mu)
+    g_e = c.gradient()
+    assert rel(g_e.real, g_o.real) < 1e-10
+    if mode == "compat":
+        assert rel(g_e.imag, g_o.imag) < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["compat", "real"])
+def test_gradient_dual_fp64(gpu_available, mode):
+    p = small_dual()
+    o, data, c, mu, _ = _setup(p, THETA2, "fp64", mode)
+    g_o = o.compute_gradient(data, o.mu)
+    g_e = c.gradient()
+    assert rel(g_e.real, g_o.real) < 1e-10
+    # energies of the injected state
+    assert abs(c.log_prior() - o.log_prior()) < 1e-9 * abs(o.log_prior())
+    assert abs(c.log_likelihood() - o.log_likelihood(data)) < 1e-9 * abs(o.log_likelihood(data))
+
+
+@pytest.mark.parametrize("dual", [False, True])
+@pytest.mark.parametrize("mode", ["compat", "real"])
+def test_leapfrog_fp64(gpu_available, dual, mode):
+    p = small_dual() if dual else small_single()
+    theta = THETA2 if dual else THETA1
+    o, data, c, mu, mom = _setup(p, theta, "fp64", mode)
+    o._compute_mass_matrix_eigen_values()
+    eps, L = 0.07, 6
+    lp0 = o._log_posterior(data, mu)
+    k0 = o._potential_energy(mom)
+    mu1, p1 = o.leapfrog(data, mu.copy(), mom.copy(), eps, L)
+    lp1 = o._log_posterior(data, mu1)
+    k1 = o._potential_energy(p1)
+    r = c.leapfrog(eps, L)
+    mu_e, p_e = c.get_state()
+    assert rel(mu_e.real, mu1.real) < 1e-9
+    assert rel(p_e.real, p1.real) < 1e-9
+    if mode == "compat":
+        assert rel(mu_e.imag, mu1.imag) < 1e-9
+    assert abs((r["log_prior1"] + r["log_lik1"]) - lp1) < 1e-9 * abs(lp1)
+    assert abs((r["log_prior0"] + r["log_lik0"]) - lp0) < 1e-9 * abs(lp0)
+    assert abs(r["k1"] - k1) < 1e-9 * abs(k1)
+    assert abs(r["e0"] - k0) < 1e-9 * abs(k0)
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_update_injected_noise_fp64(gpu_available, dual):
+    """Full iteration (momentum draw, leapfrog, energies, accept) vs the oracle."""
+    p = small_dual() if dual else small_single()
+    theta = THETA2 if dual else THETA1
+    o, data, c, mu, _ = _setup(p, theta, "fp64", "compat")
+    rng = np.random.default_rng(42)
+    zr, zi = rng.standard_normal(o.N), rng.standard_normal(o.N)
+    eps, L = 0.05, 5
+    u = 0.37
+    o.rng = mrp.ScriptedRng(normals=[(zr, zi)], uniforms=[u], gammas=[1.0, 1.0])
+    mh_o = o.update(data, eps, L)
+    r = c.update(eps, L, u, noise=(zr.reshape(o.shape), zi.reshape(o.shape)))
+    assert abs(r["H0"] - o.initial_energy) < 1e-9 * abs(o.initial_energy)
+    assert abs(r["H1"] - o.proposed_energy) < 1e-9 * abs(o.proposed_energy)
+    assert abs(r["mh_rate"] - mh_o) < 1e-6
+    assert bool(r["accepted"]) == bool(o.accepted)
+    mu_e, _ = c.get_state()
+    assert rel(mu_e.real, o.mu.real) < 1e-9
+    assert rel(mu_e.imag, o.mu.imag) < 1e-9
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_gradient_leapfrog_fp32(gpu_available, dual):
+    p = small_dual() if dual else small_single()
+    theta = THETA2 if dual else THETA1
+    o, data, c, mu, mom = _setup(p, theta, "fp32", "compat")
+    g_o = o.compute_gradient(data, o.mu)
+    g_e = c.gradient()
+    assert rel(g_e.real, g_o.real) < 2e-4
+    o._compute_mass_matrix_eigen_values()
+    eps, L = 0.07, 6
+    mu1, p1 = o.leapfrog(data, mu.copy(), mom.copy(), eps, L)
+    lp1 = o._log_posterior(data, mu1)
+    r = c.leapfrog(eps, L)
+    mu_e, p_e = c.get_state()
+    assert rel(mu_e.real, mu1.real) < 2e-3
+    assert rel(p_e.real, p1.real) < 2e-3
+    assert abs((r["log_prior1"] + r["log_lik1"]) - lp1) < 1e-4 * abs(lp1)
+
+
+def test_output_gather_and_init(gpu_available):
+    """add_data_to_mu + mu_() gather: mu_ == mu at data indices; init draw scale."""
+    p = small_single()
+    c = engine_chain(p, THETA1)
+    mu0 = c.mu()
+    c.add_data_to_mu()
+    mu1 = c.mu()
+    assert rel(mu1 - mu0, p["Yh"]) < 1e-12
+    c.set_output_indices(p["ijk"])
+    assert np.allclose(c.mu_(), mu1, rtol=0, atol=1e-12)
+
+
+def test_too_many_negative_eigenvalues(gpu_available):
+    """A constant kernel (psi = 0) has N-1 zero eigenvalues on a power-of-two
+    torus, so the constructor raises EDOMAIN (MultiResParameters.h:283-284)."""
+    from dualres_amd import DrDomainError
+    p = small_single()
+    with pytest.raises(DrDomainError):
+        engine_chain(p, (1.0, 0.0, 1.0))
