@@ -1,0 +1,244 @@
+"""Leapfrog gradient-evaluation throughput of the HMC sampler on one MI355X.
+
+Workload (the dual-resolution 1 mm fit): highres 128x160x128 at 1 mm and
+stdres 64x80x64 at 2 mm (2x2x2 block means of the same GP draw), centred
+ellipsoid masks, kernel tau2=0.806 psi=0.132 nu=1.5, kriging neighbourhood
+6.9 mm (W built on the device), circulant embedding 256x320x256, L=25.
+
+A "step" is one post-burn-in HMC iteration: momentum draw, L+1 gradient
+evaluations, energies, accept, Gibbs sigma draws and the thinned save.
+value = sum(L+1) / wall time over the timed iterations (grad evals/s).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "leapfrog grad evals/s, 1mm dual-res, FP32 vs FP64 GPU; p50/p99 fit latency"
+HBM_PEAK_GBS = 8000.0
+DIMS = (128, 160, 128)
+VOX = 1.0
+THETA = (0.806, 0.132, 1.5)
+RADIUS = 6.9
+EMBED = (256, 320, 256)
+L_STEPS = 25
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem():
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
+    t = time.perf_counter()
+    p = problem.dual_problem(DIMS, VOX, THETA, seed_gp=20261202, seed_noise=(1, 2))
+    log(f"[bench] synthetic images in {time.perf_counter() - t:.1f}s: n_h={p['Yh'].size} n_s={p['Ys'].size}")
+    t = time.perf_counter()
+    W, info = kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], THETA, RADIUS, return_info=True)
+    p["W"] = W
+    p["kriging_s"] = time.perf_counter() - t
+    p["kriging_unique"] = info["n_unique_solves"]
+    log(f"[bench] kriging W {W.shape} nnz={W.nnz} unique solves={info['n_unique_solves']} in {p['kriging_s']:.1f}s")
+    return p
+
+
+def make_chain(p, precision, mode, seed=8675309):
+    from dualres_amd import Chain
+    c = Chain(p["ijk"], THETA, p["q_hr"], n_datasets=2, precision=precision, state_mode=mode, embed=EMBED, seed=seed)
+    c.set_data(p["Yh"], p["Ys"], p["W"])
+    c.add_data_to_mu()
+    c.set_output_indices(p["ijk"])
+    return c
+
+
+def run_chain(c, p, steps, warmup, seed=8675309):
+    """warmup iterations of dual averaging, then `steps` timed post-burn-in iterations."""
+    from dualres_amd.sampler import HMCParameters, McmcSummary, gibbs_sigma, init_sigma
+    rng = np.random.default_rng(seed)
+    init_sigma(c, rng, THETA[0])
+    hmc = HMCParameters(burnin=warmup, n_save=steps, thin=1, integrator_steps=L_STEPS, starting_eps=1.0 / L_STEPS,
+                        mh_target=0.65)
+    summ = McmcSummary(c.n_out, 2)
+    n_s = p["Ys"].size
+    lat = []
+    grads = 0
+    t0 = None
+    while hmc.iteration < hmc.max_iterations():
+        timed = hmc.iteration >= warmup
+        if timed and t0 is None:
+            t0 = time.perf_counter()
+        ti = time.perf_counter()
+        eps = hmc.eps(rng.uniform(0.9, 1.1))
+        L = hmc.integrator_steps()
+        r = c.update(eps, L, rng.uniform())
+        hmc.update(r["mh_rate"])
+        s = gibbs_sigma(c, rng, c.n_h, n_s, r["sse_h"], r["sse_s"])
+        if hmc.save_iteration():
+            mu = c.mu_()
+            summ.update(mu, np.sqrt(1.0 / np.asarray(s)), -0.5 * (s[0] * r["sse_h"] + s[1] * r["sse_s"]))
+        if timed:
+            grads += L + 1
+            lat.append(time.perf_counter() - ti)
+    wall = time.perf_counter() - t0
+    return {"grad_evals_s": grads / wall, "wall": wall, "grads": grads, "ms_per_iter": 1e3 * wall / steps,
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "mh_rate": hmc.metropolis_hastings_rate(), "eps": hmc.eps_value(), "mean": summ.mode_mu()}
+
+
+def kernel_bytes(name, p, prec_bytes, c):
+    """Algorithmic HBM bytes of one launch of the named kernel (read once + write once)."""
+    s = prec_bytes
+    cs = 2 * s
+    M0, M1, M2 = EMBED
+    H0 = M0 // 2 + 1
+    Nh = H0 * M1 * M2
+    lo, hi = p["ijk"].min(0), p["ijk"].max(0)
+    n0, n1, n2 = (hi + 1).tolist()
+    nreg = n0 * n1 * n2
+    T1 = H0 * M1 * n2
+    T2 = H0 * n1 * n2
+    nnz = p["W"].nnz
+    n_s = p["Ys"].size
+    n_h = p["Yh"].size
+    compat = c.state_mode == "compat"
+    k = 2 if compat else 1  # real + imaginary chains in the pointwise spectral work
+    if name == "c2r_z_posupdate":
+        return Nh * (k * 3 * cs + s) + T1 * cs          # mu, p read, mu' write (x2 chains), invlam, T1
+    if name == "r2c_z_momentum":
+        return T1 * cs + Nh * (k * 3 * cs + s)          # T1, mu, p read, p write, invlam
+    if name in ("c2r_y", "r2c_y"):
+        return T1 * cs + T2 * cs
+    if name == "c2r_x":
+        return T2 * cs + nreg * s
+    if name == "r2c_x_residual":
+        return nreg * (3 * s + 1) + T2 * cs             # mu, Yh, W^T v, mask -> T2
+    if name == "W_mu":
+        return nnz * (4 + s) + (n_s + 1) * 4 + 2 * n_s * s + n_h * s
+    if name == "Wt_v":
+        return nnz * (4 + s) + (n_h + 1) * 4 + n_h * 4 + n_h * s + n_s * s
+    return 0
+
+
+def profile_kernels(c, p, iters=3):
+    c.profile(True)
+    for _ in range(iters):
+        c.update(0.01, L_STEPS, 1.0)
+    ks, n = c.profile_kernels()
+    c.profile(False)
+    return ks, n
+
+
+def cpu_baseline(p, workers):
+    """Complex-state oracle (reference structure: c2c FFTs, separate W and W^T)."""
+    import scipy.fft as sfft
+    from oracle import mrp
+    t = time.perf_counter()
+    o = mrp.MultiResParameters(2, THETA, p["ijk"], p["q_hr"], rng=mrp.NumpyRng(3), dtype=np.float32, M=EMBED)
+    data = mrp.MultiResData(p["Yh"], p["Ys"], p["W"], dtype=np.float32)
+    o.add_data_to_mu(data)
+    o.sigma_sq_inv = [np.float32(1.0), np.float32(4.0)]
+    o._compute_mass_matrix_eigen_values()
+    log(f"[bench] cpu oracle setup {time.perf_counter() - t:.1f}s")
+    steps = 4
+    with sfft.set_workers(workers):
+        t = time.perf_counter()
+        o.leapfrog(data, o.mu.copy(), np.zeros_like(o.mu), 1e-3, steps)
+        dt = time.perf_counter() - t
+    return {"value": (steps + 1) / dt, "unit": "grad evals/s", "cores": workers, "kind": "port",
+            "sample": f"{steps}-step leapfrog ({steps + 1} gradient evaluations, {steps} position updates) of the "
+                      f"complex-state oracle (c2c pocketfft with {workers} workers, scipy.sparse W and W^T single-"
+                      f"threaded) at the same workload, float32, {dt:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); chains and requests shard across GPUs "
+              f"without communication, so the per-GPU figure is the whole measurement.", file=sys.stderr)
+        sys.exit(2)
+    p = build_problem()
+    results = {}
+    main_chain = None
+    for prec in ("fp32", "fp64"):
+        c = make_chain(p, prec, "compat")
+        r = run_chain(c, p, a.steps, a.warmup)
+        log(f"[bench] {prec} compat: {r['grad_evals_s']:.1f} grad evals/s, {r['ms_per_iter']:.2f} ms/iter, "
+            f"mh {r['mh_rate']:.2f}, eps {r['eps']:.3g}")
+        results[prec] = r
+        if prec == a.precision:
+            main_chain = c
+            ks, nk = profile_kernels(c, p)
+        else:
+            c.close()
+    cr = make_chain(p, a.precision, "real")
+    rr = run_chain(cr, p, a.steps, a.warmup)
+    cr.close()
+    log(f"[bench] {a.precision} real: {rr['grad_evals_s']:.1f} grad evals/s")
+    rel_mean = float(np.linalg.norm(results["fp32"]["mean"] - results["fp64"]["mean"])
+                     / np.linalg.norm(results["fp64"]["mean"]))
+    # roofline of the dominant kernel
+    sb = 4 if a.precision == "fp32" else 8
+    dom = max(ks, key=ks.get)
+    byt = kernel_bytes(dom, p, sb, main_chain)
+    ach = byt / (ks[dom] * 1e-3) / 1e9
+    step_ms = sum(ks.values())
+    log("[bench] kernels (ms/step): " + ", ".join(f"{k}={v:.4f}" for k, v in ks.items()))
+    fft_names = [k for k in ks if k.startswith(("c2r", "r2c"))]
+    fft_bytes = sum(kernel_bytes(k, p, sb, main_chain) for k in fft_names)
+    fft_ms = sum(ks[k] for k in fft_names)
+    main_chain.close()
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(p, workers=min(16, os.cpu_count() or 1))
+        log(f"[bench] cpu baseline {cpu['value']:.3f} grad evals/s")
+    r = results[a.precision]
+    line = {
+        "metric": METRIC,
+        "value": round(r["grad_evals_s"], 2),
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["ms_per_iter"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.precision == "fp32" else "f64",
+        "data": "synthetic (seeded GP draw + noise; W from the device kriging build)",
+        "config": {
+            "workload": "dual-res 1mm: highres 128x160x128 @1mm + stdres 64x80x64 @2mm, embedding 256x320x256, "
+                        "L=25, neighbourhood 6.9mm, tau2=0.806 psi=0.132 nu=1.5, state mode compat",
+            "n_h": int(p["Yh"].size), "n_s": int(p["Ys"].size), "W_nnz": int(p["W"].nnz),
+            "kriging_unique_solves": int(p["kriging_unique"]), "kriging_build_s": round(p["kriging_s"], 2),
+        },
+        "fp64_value": round(results["fp64"]["grad_evals_s"], 2),
+        "fp32_real_state_value": round(rr["grad_evals_s"], 2),
+        "iteration_latency_ms": {"p50": round(r["p50_ms"], 3), "p99": round(r["p99_ms"], 3)},
+        "posterior_mean_rel_diff_fp32_vs_fp64": rel_mean,
+        "kernel_ms_per_leapfrog_step": {k: round(v, 5) for k, v in ks.items()},
+        "fft_passes_hbm_gbs": round(fft_bytes / (fft_ms * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(byt), "launch_ms": round(ks[dom], 5)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,6 +88,7 @@ SIGNATURES = {
     "dr_leapfrog": [_P, C.c_double, C.c_int, C.POINTER(dr_update_result)],
     "dr_profile_enable": [_P, C.c_int],
     "dr_profile_read": [_P, _D, _I64, _D, _D],
+    "dr_profile_kernels": [_P, _D, C.c_int, C.POINTER(C.c_int), _I64],
     "dr_kriging_build": [_D, C.POINTER(C.c_int), _F, _D, C.POINTER(C.c_int), _F, _D, C.c_double, C.c_int,
                          C.POINTER(_P)],
     "dr_csr_info": [_P, _I64, _I64, _I64, _I64],

@@ -152,6 +152,8 @@ struct ChainBase {
   virtual void gradient(double* gr, double* gi) = 0;
   virtual void refresh_stats() = 0;
   virtual void get_lambda(double* out) = 0;
+  virtual void reset_kernel_profile() = 0;
+  virtual void kernel_profile(double* ms, int max_k, int* n_k, int64_t* n_steps) = 0;
 
   long region_index(int x, int y, int z) const { return x + (long)reg[0] * (y + (long)reg[1] * z); }
   double log_lik(double sh, double ss) const {
@@ -182,10 +184,23 @@ struct Chain : public ChainBase {
   int slot_nblk[NSLOT] = {0};
   std::vector<int32_t> out_g;  // region grid index of each output voxel
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // per-kernel profile of one leapfrog step: events between launches
+  static constexpr int kMarks = 12;
+  hipEvent_t pev[kMarks] = {};
+  int rec = -1, nrec = 0;
+  double prof_k[kMarks] = {0};
+  void mark() {
+    if (rec >= 0 && rec < kMarks) {
+      DR_HIP(hipEventRecord(pev[rec], stream));
+      ++rec;
+    }
+  }
 
   ~Chain() override {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : pev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -212,6 +227,7 @@ struct Chain : public ChainBase {
     hipLaunchKernelGGL((k_strided<U, INV, Pro, Epi>), grid, dim3(kThreads), lds, stream, ad, NB, k_in, k_out, tbs,
                        pro, epi);
     DR_HIP(hipGetLastError());
+    mark();
   }
   template <typename U, class Pro, class Epi>
   void xr2c(int nrows, Pro pro, Epi epi, int slot = -1) {
@@ -224,6 +240,7 @@ struct Chain : public ChainBase {
     if (slot >= 0) note_blocks(slot, grid);
     hipLaunchKernelGGL((k_r2c_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
     DR_HIP(hipGetLastError());
+    mark();
   }
   template <typename U, class Pro, class Epi>
   void xc2r(int nrows, Pro pro, Epi epi, int slot = -1) {
@@ -236,6 +253,7 @@ struct Chain : public ChainBase {
     if (slot >= 0) note_blocks(slot, grid);
     hipLaunchKernelGGL((k_c2r_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
     DR_HIP(hipGetLastError());
+    mark();
   }
   void note_blocks(int slot, long nb) {
     DR_REQUIRE(nb <= pstride, DR_ELOGIC, "reduction grid exceeds partial buffer");
@@ -248,6 +266,7 @@ struct Chain : public ChainBase {
     DR_HIP(hipSetDevice(cfg.device));
     DR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) DR_HIP(hipEventCreate(&e));
+    for (auto& e : pev) DR_HIP(hipEventCreate(&e));
     H0 = M[0] / 2 + 1;
     N = (long)M[0] * M[1] * M[2];
     Nh = (long)H0 * M[1] * M[2];
@@ -539,10 +558,12 @@ struct Chain : public ChainBase {
     hipLaunchKernelGGL(k_wmu<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, Wrp.p, Wcols.p, Wvals.p, R, Ysd.p,
                        (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr);
     DR_HIP(hipGetLastError());
+    mark();
     const int g2 = cdiv(n_h, kThreads / 16);
     hipLaunchKernelGGL(k_wtv<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, WTrp.p, WTrows.p, WTvals.p, vsd.p,
                        gI.p, WTv.p);
     DR_HIP(hipGetLastError());
+    mark();
   }
   LoadResidual<T> residual(const T* R, int slot) {
     return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr, (T)sigma[0], reg[0],
@@ -582,9 +603,17 @@ struct Chain : public ChainBase {
       LoadPosUpdate<T> pu{{}, src, muB.p, p.p, invlam.p, sk, (T)eps, (T)sigma[0], srci,
                           compat() ? miB.p : nullptr, compat() ? pim.p : nullptr};
       if (prof && L >= 2 && s == 1) DR_HIP(hipEventRecord(ev[0], stream));
+      if (prof && L >= 3 && s == 1) {
+        rec = 0;
+        mark();
+      }
       c2r_region(pu, RB.p);
       grad_step(muB.p, compat() ? miB.p : nullptr, RB.p, (T)(k * eps), s == L - 1, S_SSEH_B, S_SSES_B, S_PRIOR_B,
                 S_PRIORI_B, S_KIN_B, S_KINI_B);
+      if (rec >= 0) {
+        nrec = rec;
+        rec = -1;
+      }
       if (prof && L >= 2 && s == L - 1) DR_HIP(hipEventRecord(ev[1], stream));
     }
   }
@@ -782,6 +811,24 @@ struct Chain : public ChainBase {
     DR_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     prof_ms += ms;
     prof_n += n;
+    for (int i = 0; i + 1 < nrec; ++i) {
+      float m = 0.f;
+      DR_HIP(hipEventElapsedTime(&m, pev[i], pev[i + 1]));
+      prof_k[i] += m;
+    }
+    if (nrec > 1) ++prof_kn;
+  }
+  long prof_kn = 0;
+  void reset_kernel_profile() override {
+    for (auto& v : prof_k) v = 0.0;
+    prof_kn = 0;
+    nrec = 0;
+  }
+  void kernel_profile(double* ms, int max_k, int* n_k, int64_t* n_steps) override {
+    const int nk = std::max(0, nrec - 1);
+    *n_k = nk;
+    *n_steps = prof_kn;
+    for (int i = 0; i < nk && i < max_k; ++i) ms[i] = prof_kn ? prof_k[i] / (double)prof_kn : 0.0;
   }
 };
 
@@ -1013,7 +1060,12 @@ int dr_profile_enable(dr_chain* c, int enable) {
     k.prof = enable != 0;
     k.prof_ms = 0.0;
     k.prof_n = 0;
+    k.reset_kernel_profile();
   });
+}
+
+int dr_profile_kernels(dr_chain* c, double* ms, int max_k, int* n_k, int64_t* n_steps) {
+  return guarded([&] { c->impl->kernel_profile(ms, max_k, n_k, n_steps); });
 }
 int dr_profile_read(dr_chain* c, double* ms_per_grad, int64_t* n_grads, double* ms_fft_passes, double* ms_w) {
   return guarded([&] {

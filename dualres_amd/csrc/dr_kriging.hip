@@ -1,0 +1,798 @@
+// Sparse local-kriging matrix W on the GPU (kriging_matrix.h:41-343).
+//
+//  1. k_pattern (one wavefront per stdres voxel): the reference's neighbour
+//     search over the offset ball P, in P order, with its flat-offset range
+//     check; emits (bounded column, offset index, k') and a 128-bit
+//     position-sensitive hash per row.
+//  2. Host: rows with equal (count, hash) share one kriging system; equality
+//     is then verified element by element on the device (k_verify).
+//  3. Unique systems, sorted by size and padded with an identity block to a
+//     multiple of 64, are factorised in batches by a blocked right-looking
+//     float64 Cholesky (k_potrf / k_trsm / k_syrk over 64x64 tiles) and
+//     solved (k_trisolve), then normalised by their sum (kriging_matrix.h:318-321).
+//  4. Rows gather their system's weights; rows whose weights sum to NaN stay
+//     empty, as in the reference.
+// K and k' entries are float32 (the reference evaluates rbf<float>), taken as
+// the float32 rounding of the float64 kernel.  Systems whose Cholesky hits a
+// nonpositive pivot (indefinite K when tau^2 > 1) are re-solved on the host
+// with partial-pivoting LU.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+#include "dr_common.hpp"
+
+namespace drh {
+
+namespace {
+
+constexpr int NB = 64;
+
+struct Geo {
+  int nx, ny, nz;
+  long nvox;
+  int lo[3], bd[3];
+  long nbidx;
+  float map[16], qs[16], qh[16];
+  double tau2, psi, nu;  // float-rounded parameters, evaluated in double
+  float eps0;
+  int nP;
+};
+
+__device__ __forceinline__ float dot4(const float* q, int r, const float* v) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc = __fadd_rn(acc, __fmul_rn(q[4 * r + c], v[c]));
+  return acc;
+}
+// host twin (x86-64 without FMA: separately rounded products and sums)
+static float dot4_h(const float* q, int r, const float* v) {
+  float acc = 0.0f;
+  for (int c = 0; c < 4; ++c) {
+    const volatile float prod = q[4 * r + c] * v[c];
+    acc = acc + prod;
+  }
+  return acc;
+}
+
+__host__ __device__ __forceinline__ float kern_f(const Geo& g, float d) {
+  return (float)(g.tau2 * exp(-g.psi * pow(fabs((double)d), g.nu)));
+}
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+// pass = 0: counts and hashes; pass = 1: write entries at rp[row]
+__global__ void __launch_bounds__(256)
+k_pattern(Geo g, int n_rows, const int* __restrict__ sijk, const int* __restrict__ P, const uint8_t* __restrict__ present,
+          const int* __restrict__ bidx, int pass, int* __restrict__ count, uint64_t* __restrict__ h1,
+          uint64_t* __restrict__ h2, int* __restrict__ oob, const long* __restrict__ rp, int* __restrict__ cols,
+          int* __restrict__ pidx, float* __restrict__ kp) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;  // whole wavefront exits together
+  float s4[4] = {(float)sijk[3 * row], (float)sijk[3 * row + 1], (float)sijk[3 * row + 2], 1.0f};
+  int base[3];
+  float xs[4];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) base[r] = (int)truncf(dot4(g.map, r, s4));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) xs[r] = dot4(g.qs, r, s4);
+  int n = 0, noob = 0;
+  uint64_t a1 = 0, a2 = 0;
+  const long wbase = pass ? rp[row] : 0;
+  for (int j0 = 0; j0 < g.nP; j0 += 64) {
+    const int j = j0 + lane;
+    bool valid = j < g.nP;
+    int hx = 0, hy = 0, hz = 0;
+    long b = -1;
+    int col = -1;
+    if (valid) {
+      hx = base[0] + P[3 * j];
+      hy = base[1] + P[3 * j + 1];
+      hz = base[2] + P[3 * j + 2];
+      const long off = (long)hz * g.ny * g.nx + (long)hy * g.nx + hx;
+      valid = off >= 0 && off < g.nvox && present[off];
+      if (valid) {
+        b = (long)(hz - g.lo[2]) * g.bd[0] * g.bd[1] + (long)(hy - g.lo[1]) * g.bd[0] + (hx - g.lo[0]);
+        if (b < 0 || b >= g.nbidx) {
+          valid = false;
+          ++noob;
+        } else {
+          col = bidx[b];
+          valid = col >= 0;
+        }
+      }
+    }
+    float k = 0.0f;
+    if (valid) {
+      const float h4[4] = {(float)hx, (float)hy, (float)hz, 1.0f};
+      float ss = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float d = __fsub_rn(xs[r], dot4(g.qh, r, h4));
+        ss = __fadd_rn(ss, __fmul_rn(d, d));
+      }
+      k = kern_f(g, __fadd_rn(sqrt_rn(ss), g.eps0));
+    }
+    const uint64_t bal = __ballot(valid);
+    const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
+    if (valid) {
+      const int pos = n + before;
+      if (pass) {
+        cols[wbase + pos] = col;
+        pidx[wbase + pos] = j;
+        kp[wbase + pos] = k;
+      } else {
+        uint32_t kb;
+        memcpy(&kb, &k, 4);
+        const uint64_t key = ((uint64_t)(uint32_t)j << 32) ^ kb;
+        a1 += mix64(key + 0x9E3779B97F4A7C15ULL * (uint64_t)(pos + 1));
+        a2 += mix64((key * 0xD6E8FEB86659FD93ULL) ^ (uint64_t)(pos + 7) * 0xA0761D6478BD642FULL);
+      }
+    }
+    n += __popcll(bal);
+  }
+  if (!pass) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a1 += __shfl_xor(a1, o, 64);
+      a2 += __shfl_xor(a2, o, 64);
+      noob += __shfl_xor(noob, o, 64);
+    }
+    if (lane == 0) {
+      count[row] = n;
+      h1[row] = a1;
+      h2[row] = a2;
+      oob[row] = noob;
+    }
+  }
+}
+
+// rows sharing a system must have identical (offset index, k') lists
+__global__ void k_verify(int n_rows, const long* __restrict__ rp, const int* __restrict__ rep,
+                         const int* __restrict__ pidx, const float* __restrict__ kp, int* __restrict__ bad) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int r = rep[row];
+  if (r == row) return;
+  const long a = rp[row], b = rp[r], n = rp[row + 1] - rp[row];
+  int mism = 0;
+  for (long t = lane; t < n; t += 64) {
+    if (pidx[a + t] != pidx[b + t]) mism = 1;
+    if (__float_as_uint(kp[a + t]) != __float_as_uint(kp[b + t])) mism = 1;
+  }
+  const uint64_t bal = __ballot(mism != 0);
+  if (lane == 0 && bal) bad[row] = 1;
+}
+
+struct SysDesc {
+  long off;   // entry offset of the representative row
+  int n;      // system size
+  long sol;   // offset into the solution buffer
+};
+
+// A_s = [K[pidx, pidx] 0; 0 I] padded to np; b_s = [k'; 0]
+__global__ void k_assemble(const SysDesc* __restrict__ sd, int np, const float* __restrict__ K, int nP,
+                           const int* __restrict__ pidx, const float* __restrict__ kp, double* __restrict__ A,
+                           double* __restrict__ bv) {
+  const int s = blockIdx.y;
+  const SysDesc d = sd[s];
+  const long npp = (long)np * np;
+  double* As = A + (long)s * npp;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npp; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / np), j = (int)(t - (long)i * np);
+    double v;
+    if (i < d.n && j < d.n) v = (double)K[(long)pidx[d.off + i] * nP + pidx[d.off + j]];
+    else v = (i == j) ? 1.0 : 0.0;
+    As[t] = v;
+  }
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < np; i += blockDim.x) bv[(long)s * np + i] = i < d.n ? (double)kp[d.off + i] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_potrf(double* A, int np, int k, int* fail) {
+  __shared__ double L[NB][NB + 1];
+  double* As = A + (long)blockIdx.x * np * np;
+  const int o = k * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    L[i][j] = As[(long)(o + i) * np + o + j];
+  }
+  __syncthreads();
+  for (int j = 0; j < NB; ++j) {
+    const double d = L[j][j];
+    __syncthreads();
+    const double sd = sqrt(d);
+    if (threadIdx.x == 0) {
+      L[j][j] = sd;
+      if (!(d > 0.0)) fail[blockIdx.x] = 1;
+    }
+    for (int i = j + 1 + threadIdx.x; i < NB; i += 256) L[i][j] /= sd;
+    __syncthreads();
+    const int m = NB - j - 1;
+    for (int t = threadIdx.x; t < m * m; t += 256) {
+      const int ii = j + 1 + t / m, ll = j + 1 + t % m;
+      if (ll <= ii) L[ii][ll] -= L[ii][j] * L[ll][j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    if (j <= i) As[(long)(o + i) * np + o + j] = L[i][j];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_trsm(double* A, int np, int k) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  double* As = A + (long)blockIdx.x * np * np;
+  const int o = k * NB;
+  const int r0 = (k + 1 + blockIdx.y) * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    L[i][j] = As[(long)(o + i) * np + o + j];
+    X[i][j] = As[(long)(r0 + i) * np + o + j];
+  }
+  __syncthreads();
+  for (int j = 0; j < NB; ++j) {
+    const double d = L[j][j];
+    for (int i = threadIdx.x; i < NB; i += 256) X[i][j] /= d;
+    __syncthreads();
+    const int m = NB - j - 1;
+    for (int t = threadIdx.x; t < NB * m; t += 256) {
+      const int i = t / m, l = j + 1 + t % m;
+      X[i][l] -= X[i][j] * L[l][j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    As[(long)(r0 + i) * np + o + j] = X[i][j];
+  }
+}
+
+// trailing update of lower tiles: A[BI][BJ] -= P[BI] P[BJ]^T
+__global__ void __launch_bounds__(256) k_syrk(double* A, int np, int k) {
+  __shared__ double Pi[NB][NB + 1];
+  __shared__ double Pj[NB][NB + 1];
+  double* As = A + (long)blockIdx.x * np * np;
+  const int t = blockIdx.y;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int BI = (k + 1 + bi) * NB, BJ = (k + 1 + bj) * NB, o = k * NB;
+  for (int q = threadIdx.x; q < NB * NB; q += 256) {
+    const int i = q / NB, l = q % NB;
+    Pi[i][l] = As[(long)(BI + i) * np + o + l];
+    Pj[i][l] = As[(long)(BJ + i) * np + o + l];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+  for (int l = 0; l < NB; ++l) {
+    double a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = Pi[ty * 4 + r][l];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = Pj[tx * 4 + c][l];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) As[(long)(BI + ty * 4 + r) * np + BJ + tx * 4 + c] -= acc[r][c];
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// L L^T w = b, then w /= sum(w) (sum 0 -> 1); writes n weights to sol
+__global__ void __launch_bounds__(256) k_trisolve(const double* A, int np, const SysDesc* __restrict__ sd,
+                                                  const double* __restrict__ bv, double* __restrict__ sol,
+                                                  int* __restrict__ nanflag, int s0) {
+  extern __shared__ double y[];
+  __shared__ double part[4];
+  const int s = blockIdx.x;
+  const SysDesc d = sd[s];
+  const double* As = A + (long)s * np * np;
+  const int n = d.n;
+  for (int i = threadIdx.x; i < n; i += 256) y[i] = bv[(long)s * np + i];
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int j = 0; j < n; ++j) {
+    const double* Lj = As + (long)j * np;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < j; i += 256) acc += Lj[i] * y[i];
+    acc = wsum(acc);
+    if (lane == 0) part[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) y[j] = (y[j] - ((part[0] + part[1]) + (part[2] + part[3]))) / Lj[j];
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    const double* Lj = As + (long)j * np;
+    const double wj = y[j] / Lj[j];
+    __syncthreads();
+    if (threadIdx.x == 0) y[j] = wj;
+    for (int i = threadIdx.x; i < j; i += 256) y[i] -= Lj[i] * wj;
+    __syncthreads();
+  }
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += y[i];
+  acc = wsum(acc);
+  if (lane == 0) part[w] = acc;
+  __syncthreads();
+  double tot = (part[0] + part[1]) + (part[2] + part[3]);
+  if (tot == 0.0) tot = 1.0;
+  if (threadIdx.x == 0 && isnan(tot)) nanflag[s0 + s] = 1;
+  for (int i = threadIdx.x; i < n; i += 256) sol[d.sol + i] = y[i] / tot;
+}
+
+// vals[e] = sol[sys_sol[uid(row)] + (e - rp[row])]
+__global__ void k_gather_rows(int n_rows, const long* __restrict__ rp, const int* __restrict__ uid,
+                              const long* __restrict__ usol, const double* __restrict__ sol, double* __restrict__ vals) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const long a = rp[row], n = rp[row + 1] - a, so = usol[uid[row]];
+  for (long t = lane; t < n; t += 64) vals[a + t] = sol[so + t];
+}
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  DBuf() = default;
+  explicit DBuf(size_t n) { alloc(n); }
+  DBuf(const DBuf&) = delete;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void**)&p, n * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      throw Error(DR_ENOMEM, "hipMalloc failed in kriging build");
+    }
+  }
+  void up(const T* h, size_t n, hipStream_t s) { DR_HIP(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s)); }
+  void down(T* h, size_t n, hipStream_t s) { DR_HIP(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s)); }
+};
+
+// partial-pivoting LU solve (host fallback for indefinite systems)
+bool lu_solve(std::vector<double>& A, std::vector<double>& b, int n) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double m = std::fabs(A[(long)k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[(long)i * n + k]) > m) { m = std::fabs(A[(long)i * n + k]); p = i; }
+    if (m == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[(long)k * n + j], A[(long)p * n + j]);
+      std::swap(b[k], b[p]);
+    }
+    const double d = A[(long)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[(long)i * n + k] / d;
+      if (f == 0.0) continue;
+      for (int j = k + 1; j < n; ++j) A[(long)i * n + j] -= f * A[(long)k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[(long)i * n + j] * b[j];
+    b[i] = s / A[(long)i * n + i];
+  }
+  return true;
+}
+
+float fsq(float a) {
+  const volatile float r = a * a;
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host build
+struct KrigingCSR {
+  long nrow = 0, ncol = 0, n_unique = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> cols;
+  std::vector<double> vals;
+};
+
+static float norm3f(const float* v) {
+  float s = 0.0f;
+  for (int c = 0; c < 3; ++c) {
+    const volatile float sq = v[c] * v[c];
+    s = s + sq;
+  }
+  return sqrt_rn(s);
+}
+
+KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], const double* st, const int sdims[3],
+                         const float qs[16], const double theta[3], double radius, int device) {
+  DR_REQUIRE(hr && st && hd && sdims && qh && qs && theta, DR_ELOGIC, "kriging_build: null argument");
+  DR_REQUIRE(radius >= 0, DR_EDOMAIN, "neighborhood_perturbation: radius cannot be negative");
+  DR_HIP(hipSetDevice(device));
+  const int nx = hd[0], ny = hd[1], nz = hd[2];
+  const long nvox = (long)nx * ny * nz;
+  // neighbourhood offsets (kriging_matrix.h:63-106)
+  float vd[3];
+  for (int c = 0; c < 3; ++c) {
+    float s = 0.0f;
+    for (int r = 0; r < 4; ++r) s += qh[4 * r + c] * qh[4 * r + c];
+    vd[c] = sqrt_rn(s);
+  }
+  const float rad = (float)radius;
+  int rng[3];
+  for (int c = 0; c < 3; ++c) rng[c] = (int)(rad / vd[c]);
+  std::vector<int> P;
+  std::vector<float> X;
+  for (int z = -rng[2]; z <= rng[2]; ++z)
+    for (int y = -rng[1]; y <= rng[1]; ++y)
+      for (int x = -rng[0]; x <= rng[0]; ++x) {
+        const float p4[4] = {(float)x, (float)y, (float)z, 0.0f};
+        float xx[3];
+        for (int r = 0; r < 3; ++r) xx[r] = dot4_h(qh, r, p4);
+        if (norm3f(xx) <= rad) {
+          P.insert(P.end(), {x, y, z});
+          X.insert(X.end(), {xx[0], xx[1], xx[2]});
+        }
+      }
+  const int nP = (int)(P.size() / 3);
+  DR_REQUIRE(nP > 0, DR_ELOGIC, "neighborhood_perturbation: no perturbations within radius");
+  DR_REQUIRE(nP <= 5000, DR_ELOGIC, "get_sparse_kriging_array_data: neighborhood >5e3 voxels");
+  Geo g{};
+  g.tau2 = (double)(float)theta[0];
+  g.psi = (double)(float)theta[1];
+  g.nu = (double)(float)theta[2];
+  g.eps0 = 1e-5f;
+  g.nP = nP;
+  // K (kriging_matrix.h:111-138): unit diagonal
+  std::vector<float> K((size_t)nP * nP);
+  for (int i = 0; i < nP; ++i) {
+    K[(size_t)i * nP + i] = 1.0f;
+    for (int j = 0; j < i; ++j) {
+      const float d[3] = {X[3 * i] - X[3 * j], X[3 * i + 1] - X[3 * j + 1], X[3 * i + 2] - X[3 * j + 2]};
+      const float ss = (fsq(d[0]) + fsq(d[1])) + fsq(d[2]);
+      const float v = kern_f(g, sqrt_rn(ss));
+      K[(size_t)i * nP + j] = v;
+      K[(size_t)j * nP + i] = v;
+    }
+  }
+  // highres bounding box and bounded nonzero index (nifti_manipulation.inl:210-275)
+  auto pres = [](double v) { return !(std::isnan(v) || v == 0.0); };
+  std::vector<uint8_t> present(nvox);
+  int lo[3] = {nx, ny, nz}, hi[3] = {-1, -1, -1};
+  long nnz_hr = 0;
+  for (long f = 0; f < nvox; ++f) {
+    present[f] = pres(hr[f]);
+    if (present[f]) {
+      const int c[3] = {(int)(f % nx), (int)((f / nx) % ny), (int)(f / ((long)nx * ny))};
+      for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
+      ++nnz_hr;
+    }
+  }
+  DR_REQUIRE(nnz_hr > 0, DR_EDOMAIN, "kriging_build: high-resolution image has no nonzero voxels");
+  int bd[3];
+  for (int a = 0; a < 3; ++a) bd[a] = hi[a] - lo[a] + 1;
+  const long nb = (long)bd[0] * bd[1] * bd[2];
+  std::vector<int> bidx(nb);
+  {
+    int cnt = 0;
+    for (int k = 0; k < bd[2]; ++k)
+      for (int j = 0; j < bd[1]; ++j)
+        for (int i = 0; i < bd[0]; ++i) {
+          const long f = (long)(k + lo[2]) * nx * ny + (long)(j + lo[1]) * nx + (i + lo[0]);
+          bidx[(long)k * bd[0] * bd[1] + (long)j * bd[0] + i] = present[f] ? cnt++ : -1;
+        }
+  }
+  // stdres nonzero ijk (nifti_manipulation.inl:334-360)
+  const long snvox = (long)sdims[0] * sdims[1] * sdims[2];
+  std::vector<int> sijk;
+  for (long f = 0; f < snvox; ++f)
+    if (pres(st[f])) {
+      sijk.push_back((int)(f % sdims[0]));
+      sijk.push_back((int)((f / sdims[0]) % sdims[1]));
+      sijk.push_back((int)(f / ((long)sdims[0] * sdims[1])));
+    }
+  const int n_rows = (int)(sijk.size() / 3);
+  DR_REQUIRE(n_rows > 0, DR_EDOMAIN, "kriging_build: standard-resolution image has no nonzero voxels");
+  // map = Qh^-1 Qs (float32 rounding of the float64 product)
+  {
+    double a[16], inv[16];
+    for (int i = 0; i < 16; ++i) a[i] = qh[i];
+    // Gauss-Jordan inverse of the 4x4
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? a[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+      int p = c;
+      for (int r = c + 1; r < 4; ++r)
+        if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+      DR_REQUIRE(m[p][c] != 0.0, DR_EDOMAIN, "kriging_build: singular high-resolution qform");
+      for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[p][j]);
+      const double d = m[c][c];
+      for (int j = 0; j < 8; ++j) m[c][j] /= d;
+      for (int r = 0; r < 4; ++r)
+        if (r != c) {
+          const double f = m[r][c];
+          for (int j = 0; j < 8; ++j) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) inv[4 * i + j] = m[i][4 + j];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 4; ++k) s += inv[4 * i + k] * (double)qs[4 * k + j];
+        g.map[4 * i + j] = (float)s;
+      }
+  }
+  for (int i = 0; i < 16; ++i) { g.qs[i] = qs[i]; g.qh[i] = qh[i]; }
+  g.nx = nx; g.ny = ny; g.nz = nz; g.nvox = nvox;
+  for (int a = 0; a < 3; ++a) { g.lo[a] = lo[a]; g.bd[a] = bd[a]; }
+  g.nbidx = nb;
+
+  hipStream_t s;
+  DR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct SG { hipStream_t s; ~SG() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{s};
+  DBuf<int> d_sijk(sijk.size()), d_P(P.size()), d_bidx(nb), d_count(n_rows), d_oob(n_rows);
+  DBuf<uint8_t> d_pres(nvox);
+  DBuf<uint64_t> d_h1(n_rows), d_h2(n_rows);
+  d_sijk.up(sijk.data(), sijk.size(), s);
+  d_P.up(P.data(), P.size(), s);
+  d_bidx.up(bidx.data(), nb, s);
+  d_pres.up(present.data(), nvox, s);
+  const int gblk = cdiv(n_rows, 4);
+  hipLaunchKernelGGL(k_pattern, dim3(gblk), dim3(256), 0, s, g, n_rows, d_sijk.p, d_P.p, d_pres.p, d_bidx.p, 0,
+                     d_count.p, d_h1.p, d_h2.p, d_oob.p, (const long*)nullptr, (int*)nullptr, (int*)nullptr,
+                     (float*)nullptr);
+  DR_HIP(hipGetLastError());
+  std::vector<int> cnt(n_rows), oobv(n_rows);
+  std::vector<uint64_t> h1(n_rows), h2(n_rows);
+  d_count.down(cnt.data(), n_rows, s);
+  d_h1.down(h1.data(), n_rows, s);
+  d_h2.down(h2.data(), n_rows, s);
+  d_oob.down(oobv.data(), n_rows, s);
+  DR_HIP(hipStreamSynchronize(s));
+  std::vector<long> rp(n_rows + 1, 0);
+  for (int i = 0; i < n_rows; ++i) rp[i + 1] = rp[i] + cnt[i];
+  const long nnz = rp[n_rows];
+  DR_REQUIRE(nnz < (1L << 31), DR_EDOMAIN, "kriging_build: too many nonzeros");
+  // dedup
+  struct Key { uint64_t a, b; int n; bool operator==(const Key& o) const { return a == o.a && b == o.b && n == o.n; } };
+  struct KH { size_t operator()(const Key& k) const { return (size_t)(k.a ^ (k.b * 31) ^ (uint64_t)k.n); } };
+  std::unordered_map<Key, int, KH> seen;
+  std::vector<int> rep(n_rows), uid(n_rows), urow;
+  for (int i = 0; i < n_rows; ++i) {
+    if (cnt[i] == 0) { rep[i] = i; uid[i] = -1; continue; }
+    Key k{h1[i], h2[i], cnt[i]};
+    auto it = seen.find(k);
+    if (it == seen.end()) {
+      const int u = (int)urow.size();
+      seen.emplace(k, u);
+      urow.push_back(i);
+      rep[i] = i;
+      uid[i] = u;
+    } else {
+      uid[i] = it->second;
+      rep[i] = urow[it->second];
+    }
+  }
+  const int nU = (int)urow.size();
+  DBuf<long> d_rp(n_rows + 1);
+  DBuf<int> d_cols(nnz), d_pidx(nnz), d_rep(n_rows), d_bad(n_rows), d_uid(n_rows);
+  DBuf<float> d_kp(nnz);
+  d_rp.up(rp.data(), n_rows + 1, s);
+  d_rep.up(rep.data(), n_rows, s);
+  DR_HIP(hipMemsetAsync(d_bad.p, 0, n_rows * sizeof(int), s));
+  hipLaunchKernelGGL(k_pattern, dim3(gblk), dim3(256), 0, s, g, n_rows, d_sijk.p, d_P.p, d_pres.p, d_bidx.p, 1,
+                     (int*)nullptr, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr, d_rp.p, d_cols.p,
+                     d_pidx.p, d_kp.p);
+  DR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_verify, dim3(gblk), dim3(256), 0, s, n_rows, d_rp.p, d_rep.p, d_pidx.p, d_kp.p, d_bad.p);
+  DR_HIP(hipGetLastError());
+  std::vector<int> bad(n_rows);
+  d_bad.down(bad.data(), n_rows, s);
+  DR_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n_rows; ++i)
+    DR_REQUIRE(!bad[i], DR_ERUNTIME, "kriging_build: hash collision between distinct kriging systems");
+
+  // unique systems sorted by size, batched by padded size
+  std::vector<int> order(nU);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return cnt[urow[a]] > cnt[urow[b]]; });
+  std::vector<long> usol(nU);
+  long solsz = 0;
+  for (int u = 0; u < nU; ++u) { usol[u] = solsz; solsz += cnt[urow[u]]; }
+  DBuf<double> d_sol(solsz);
+  DBuf<float> d_K((size_t)nP * nP);
+  d_K.up(K.data(), (size_t)nP * nP, s);
+  std::vector<int> failed_u, nanf(nU, 0);
+  DBuf<int> d_nan(nU);
+  DR_HIP(hipMemsetAsync(d_nan.p, 0, nU * sizeof(int), s));
+  const double budget = 6.0e9;  // bytes of factor scratch per batch
+  size_t pos = 0;
+  DBuf<double> d_A, d_b;
+  DBuf<SysDesc> d_sd;
+  DBuf<int> d_fail;
+  size_t capA = 0, capB = 0, capS = 0;
+  while (pos < (size_t)nU) {
+    const int n0 = cnt[urow[order[pos]]];
+    const int np = ((n0 + NB - 1) / NB) * NB;
+    const long per = (long)np * np * 8;
+    const int bmax = (int)std::max<long>(1, std::min<long>(4096, (long)(budget / per)));
+    std::vector<SysDesc> sd;
+    std::vector<int> bu;
+    while (pos < (size_t)nU && (int)sd.size() < bmax) {
+      const int u = order[pos];
+      sd.push_back(SysDesc{rp[urow[u]], cnt[urow[u]], usol[u]});
+      bu.push_back(u);
+      ++pos;
+    }
+    const int B = (int)sd.size();
+    if ((size_t)B * np * np > capA) { capA = (size_t)B * np * np; d_A.alloc(capA); }
+    if ((size_t)B * np > capB) { capB = (size_t)B * np; d_b.alloc(capB); }
+    if ((size_t)B > capS) { capS = B; d_sd.alloc(capS); d_fail.alloc(capS); }
+    d_sd.up(sd.data(), B, s);
+    DR_HIP(hipMemsetAsync(d_fail.p, 0, B * sizeof(int), s));
+    const int gx = std::max(1, std::min(cdiv((long)np * np, 256), 256));
+    hipLaunchKernelGGL(k_assemble, dim3(gx, B), dim3(256), 0, s, d_sd.p, np, d_K.p, nP, d_pidx.p, d_kp.p, d_A.p,
+                       d_b.p);
+    DR_HIP(hipGetLastError());
+    const int nt = np / NB;
+    for (int k = 0; k < nt; ++k) {
+      hipLaunchKernelGGL(k_potrf, dim3(B), dim3(256), 0, s, d_A.p, np, k, d_fail.p);
+      const int m = nt - k - 1;
+      if (m > 0) {
+        hipLaunchKernelGGL(k_trsm, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
+        hipLaunchKernelGGL(k_syrk, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
+      }
+      DR_HIP(hipGetLastError());
+    }
+    std::vector<int> fail(B);
+    hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p, d_sol.p,
+                       d_nan.p, 0);
+    DR_HIP(hipGetLastError());
+    (void)bu;
+    d_fail.down(fail.data(), B, s);
+    DR_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < B; ++i)
+      if (fail[i]) failed_u.push_back(bu[i]);
+  }
+  // indefinite systems: host LU
+  if (!failed_u.empty()) {
+    std::vector<int> pidx_h(nnz);
+    std::vector<float> kp_h(nnz);
+    d_pidx.down(pidx_h.data(), nnz, s);
+    d_kp.down(kp_h.data(), nnz, s);
+    DR_HIP(hipStreamSynchronize(s));
+    for (int u : failed_u) {
+      const long off = rp[urow[u]];
+      const int n = cnt[urow[u]];
+      std::vector<double> A((size_t)n * n), b(n);
+      for (int i = 0; i < n; ++i) {
+        b[i] = kp_h[off + i];
+        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = K[(size_t)pidx_h[off + i] * nP + pidx_h[off + j]];
+      }
+      std::vector<double> w(n, std::nan(""));
+      if (lu_solve(A, b, n)) w = b;
+      double sum = 0.0;
+      for (double v : w) sum += v;
+      if (sum == 0.0) sum = 1.0;
+      for (double& v : w) v /= sum;
+      DR_HIP(hipMemcpyAsync(d_sol.p + usol[u], w.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+      DR_HIP(hipStreamSynchronize(s));
+      int nf = std::isnan(sum) ? 1 : 0;
+      DR_HIP(hipMemcpyAsync(d_nan.p + u, &nf, sizeof(int), hipMemcpyHostToDevice, s));
+      DR_HIP(hipStreamSynchronize(s));
+    }
+  }
+  // NaN systems are stored at the position their batch had; map back by unique id
+  // (k_trisolve flags by batch slot, so re-derive per unique from the solution)
+  std::vector<int> uidd(n_rows);
+  for (int i = 0; i < n_rows; ++i) uidd[i] = uid[i] < 0 ? 0 : uid[i];
+  d_uid.up(uidd.data(), n_rows, s);
+  DBuf<long> d_usol(std::max(nU, 1));
+  if (nU) d_usol.up(usol.data(), nU, s);
+  DBuf<double> d_vals(nnz);
+  hipLaunchKernelGGL(k_gather_rows, dim3(gblk), dim3(256), 0, s, n_rows, d_rp.p, d_uid.p, d_usol.p, d_sol.p,
+                     d_vals.p);
+  DR_HIP(hipGetLastError());
+  KrigingCSR out;
+  std::vector<int32_t> cols(nnz);
+  std::vector<double> vals(nnz);
+  d_cols.down(cols.data(), nnz, s);
+  d_vals.down(vals.data(), nnz, s);
+  DR_HIP(hipStreamSynchronize(s));
+  // drop rows whose weight sum is NaN (kriging_matrix.h:320-327)
+  out.nrow = n_rows;
+  out.ncol = nnz_hr;
+  out.n_unique = nU;
+  out.rp.assign(n_rows + 1, 0);
+  out.cols.reserve(nnz);
+  out.vals.reserve(nnz);
+  for (int i = 0; i < n_rows; ++i) {
+    bool keep = cnt[i] > 0;
+    if (keep) {
+      for (long e = rp[i]; e < rp[i + 1]; ++e)
+        if (std::isnan(vals[e])) { keep = false; break; }
+    }
+    if (keep) {
+      out.cols.insert(out.cols.end(), cols.begin() + rp[i], cols.begin() + rp[i + 1]);
+      out.vals.insert(out.vals.end(), vals.begin() + rp[i], vals.begin() + rp[i + 1]);
+    }
+    out.rp[i + 1] = (int64_t)out.cols.size();
+  }
+  (void)oobv;
+  (void)nanf;
+  return out;
+}
+
+}  // namespace drh
+
+struct dr_csr {
+  drh::KrigingCSR m;
+};
+
+extern "C" {
+
+int dr_kriging_build(const double* hr, const int hr_dims[3], const float q_hr[16], const double* std_img,
+                     const int std_dims[3], const float q_std[16], const double theta[3], double radius, int device,
+                     dr_csr** out) {
+  return drh::guarded([&] {
+    DR_REQUIRE(out, DR_ELOGIC, "dr_kriging_build: null output");
+    auto* c = new dr_csr{drh::kriging_build(hr, hr_dims, q_hr, std_img, std_dims, q_std, theta, radius, device)};
+    *out = c;
+  });
+}
+
+int dr_csr_info(dr_csr* m, int64_t* nrow, int64_t* ncol, int64_t* nnz, int64_t* n_unique) {
+  return drh::guarded([&] {
+    DR_REQUIRE(m, DR_ELOGIC, "null csr");
+    *nrow = m->m.nrow;
+    *ncol = m->m.ncol;
+    *nnz = (int64_t)m->m.cols.size();
+    if (n_unique) *n_unique = m->m.n_unique;
+  });
+}
+
+int dr_csr_copy(dr_csr* m, int64_t* rowptr, int32_t* cols, double* vals) {
+  return drh::guarded([&] {
+    DR_REQUIRE(m, DR_ELOGIC, "null csr");
+    std::copy(m->m.rp.begin(), m->m.rp.end(), rowptr);
+    std::copy(m->m.cols.begin(), m->m.cols.end(), cols);
+    std::copy(m->m.vals.begin(), m->m.vals.end(), vals);
+  });
+}
+
+int dr_csr_free(dr_csr* m) {
+  return drh::guarded([&] { delete m; });
+}
+
+}  // extern "C"

@@ -186,6 +186,17 @@ class Chain:
     def profile(self, enable=True):
         L.check(self._lib.dr_profile_enable(self._h, 1 if enable else 0))
 
+    def profile_kernels(self):
+        """Per-kernel ms of one leapfrog step (launch order) and the step count."""
+        ms = np.zeros(16)
+        nk, ns = C.c_int(), C.c_int64()
+        L.check(self._lib.dr_profile_kernels(self._h, L.dptr(ms), 16, C.byref(nk), C.byref(ns)))
+        names = ["c2r_z_posupdate", "c2r_y", "c2r_x"]
+        if self.n_datasets == 2:
+            names += ["W_mu", "Wt_v"]
+        names += ["r2c_x_residual", "r2c_y", "r2c_z_momentum"]
+        return dict(zip(names, ms[: nk.value].tolist())), ns.value
+
     def profile_read(self):
         ms, n, a, b = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
         L.check(self._lib.dr_profile_read(self._h, C.byref(ms), C.byref(n), C.byref(a), C.byref(b)))

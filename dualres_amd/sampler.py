@@ -1,0 +1,188 @@
+"""Host-side sampler driver over a device chain.
+
+Mirrors the reference's scalar control around the hot path:
+  * HMCParameters: dual-averaging step size (include/dualres/HMCParameters.h:97-297)
+  * mcmc_summary: running moments (include/dualres/gaussian_process_model.h:191-317)
+  * fit: the fit_model loop (include/dualres/gaussian_process_model.h:87-177)
+The host draws the scalar random numbers the reference draws from its global
+std::mt19937 (eps jitter, accept uniform, Gibbs gammas); the bulk normals come
+from the device Philox stream.
+"""
+import math
+import time
+
+import numpy as np
+
+
+class HMCParameters:
+    """Dual averaging of Hoffman & Gelman (2014) as modified by the reference."""
+
+    def __init__(self, burnin=1000, n_save=500, thin=1, integrator_steps=10, starting_eps=0.1,
+                 mh_target=0.65, gamma=0.05, t0=10.0, kappa=0.75):
+        if burnin < 0:
+            raise ValueError("HMCParameters: burnin must be >= 0")
+        if n_save < 0:
+            raise ValueError("HMCParameters: n_save must be >= 0")
+        if thin < 0:
+            raise ValueError("HMCParameters: thin must be >= 0")
+        if integrator_steps <= 0:
+            raise ValueError("HMCParameters: integrator_steps must be > 0")
+        if starting_eps <= 0:
+            raise ValueError("HMCParameters: starting_eps must be > 0")
+        if not 0 < mh_target < 1:
+            raise ValueError("HMCParameters: mh_target should be within (0, 1)")
+        if not 0 < gamma < 1:
+            raise ValueError("HMCParameters: gamma should be within (0, 1)")
+        if t0 < 0:
+            raise ValueError("HMCParameters: t0 should be >= 0")
+        if not 0 < kappa < 1:
+            raise ValueError("HMCParameters: kappa should be within (0, 1)")
+        self._warmup = burnin > 0
+        self._eps_start_found = not self._warmup
+        self.burnin = burnin
+        self._L = integrator_steps
+        self.iteration = 0
+        self.n_save = n_save
+        self.thin = 1 if thin == 0 else thin
+        self._A = 1.0
+        self._eps0 = 1e-5
+        self._eps = max(float(starting_eps), self._eps0)
+        self._eps_bar = 1.0
+        self._gamma = gamma
+        self._h_bar = 0.0
+        self._mh_sum = 0.0
+        self._mh_target = mh_target
+        self._t0 = t0
+        self._kappa = kappa
+        self._eps_target = 0.0
+
+    def save_iteration(self):
+        return self.iteration > self.burnin and (self.iteration - self.burnin) % self.thin == 0
+
+    def integrator_steps(self):
+        return self._L if self._eps_start_found else 1
+
+    def max_iterations(self):
+        return self.burnin + self.thin * self.n_save
+
+    def eps(self, u09_11):
+        if self._warmup:
+            return self._eps
+        return max(self._eps * u09_11, self._eps0)
+
+    def eps_value(self):
+        return self._eps
+
+    def metropolis_hastings_rate(self):
+        return self._mh_sum / max(self.iteration - self.burnin, 1)
+
+    def update(self, mh_rate):
+        if self.iteration == self.burnin:
+            self._warmup = False
+            self._eps_start_found = True
+            if self._eps_bar > self._eps0:
+                self._eps = self._eps_bar
+        if self._warmup:
+            if not self._eps_start_found:
+                if self.iteration == 0:
+                    self._A = 2 * int(mh_rate > 0.5) - 1
+                if math.pow(mh_rate, self._A) > math.pow(2.0, -self._A):
+                    self._eps *= math.pow(2.0, self._A)
+                else:
+                    self._eps_start_found = True
+                    self._eps_target = math.log(10.0 * self._eps)
+            else:
+                it = self.iteration
+                self._h_bar = (1 - 1 / (it + self._t0)) * self._h_bar + (self._mh_target - mh_rate) / (it + self._t0)
+                self._eps = max(math.exp(self._eps_target - math.sqrt(it) * self._h_bar / self._gamma), self._eps0)
+                a = math.pow(it, -self._kappa)
+                self._eps_bar = math.exp(a * math.log(self._eps) + (1 - a) * math.log(self._eps_bar))
+        else:
+            self._mh_sum += mh_rate
+        self.iteration += 1
+
+
+class McmcSummary:
+    """Running posterior moments (gaussian_process_model.h:191-302), float64."""
+
+    def __init__(self, n_mu, n_sigma):
+        self.samples = 0
+        self.m1 = np.zeros(n_mu)
+        self.m1_abs = np.zeros(n_mu)
+        self.m2 = np.zeros(n_mu)
+        self.sigma = np.zeros(n_sigma)
+        self.log_lik = 0.0
+
+    def update(self, mu, sigma, log_lik):
+        self.m1 += mu
+        self.m1_abs += np.abs(mu) / max(np.abs(mu).max(), 1e-6)
+        self.m2 += mu * mu
+        self.sigma += sigma
+        self.log_lik += log_lik
+        self.samples += 1
+
+    def _n(self):
+        return max(self.samples, 1)
+
+    def mode_mu(self):
+        return self.m1 / self._n()
+
+    def var_mu(self):
+        return (self.m2 - self.m1 * self.m1 / self._n()) / self._n()
+
+    def activation(self):
+        m = self.m1_abs / (self._n() * np.sqrt(self.var_mu()))
+        return m / m.max()
+
+
+def gibbs_sigma(chain, rng, n_h, n_s, sse_h, sse_s):
+    """_update_sigma_sq_inv (MultiResParameters.h:820-859)."""
+    s = [rng.gamma(0.5 * n_h + 1, 1.0 / (0.5 * sse_h))]
+    if chain.n_datasets == 2:
+        s.append(rng.gamma(0.5 * n_s + 1, 1.0 / (0.5 * sse_s)))
+    chain.set_sigma_sq_inv(s)
+    return s
+
+
+def init_sigma(chain, rng, tau2):
+    """_initialize_sigma (MultiResParameters.h:764-778)."""
+    s = [rng.gamma(tau2, 1.0)]
+    if chain.n_datasets == 2:
+        s.append(s[0] / rng.uniform())
+    chain.set_sigma_sq_inv(s)
+    return s
+
+
+def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None):
+    """fit_model loop (gaussian_process_model.h:134-171) on a prepared chain.
+
+    The chain must have its data, mu (with data added) and output indices
+    set.  Returns (McmcSummary, stats dict).
+    """
+    rng = np.random.default_rng(seed)
+    init_sigma(chain, rng, tau2)
+    summ = McmcSummary(chain.n_out, chain.n_datasets)
+    save_count = 0
+    t_start = time.perf_counter()
+    grads = 0
+    while save_count < hmc.n_save and hmc.iteration <= hmc.max_iterations():
+        eps = hmc.eps(rng.uniform(0.9, 1.1))
+        L = hmc.integrator_steps()
+        u = rng.uniform()
+        r = chain.update(eps, L, u)
+        grads += L + 1
+        hmc.update(r["mh_rate"])
+        s = gibbs_sigma(chain, rng, chain.n_h, n_s, r["sse_h"], r["sse_s"])
+        if hmc.save_iteration():
+            if save_count == 0:
+                t_start = time.perf_counter()
+                grads = 0
+            mu = chain.mu_()
+            ll = -0.5 * (s[0] * r["sse_h"] + (s[1] * r["sse_s"] if chain.n_datasets == 2 else 0.0))
+            summ.update(mu, np.sqrt(1.0 / np.asarray(s)), ll)
+            save_count += 1
+        if on_iteration:
+            on_iteration(hmc, r)
+    t = time.perf_counter() - t_start
+    return summ, {"sampling_time": t, "mh_rate": hmc.metropolis_hastings_rate(), "eps": hmc.eps_value(),
+                  "post_burnin_grad_evals": grads}

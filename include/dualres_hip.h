@@ -152,6 +152,9 @@ int dr_leapfrog(dr_chain* c, double eps, int L, dr_update_result* res);
 int dr_profile_enable(dr_chain* c, int enable);
 int dr_profile_read(dr_chain* c, double* ms_per_grad, int64_t* n_grads,
                     double* ms_fft_passes, double* ms_w);
+/* Per-kernel average device time (ms) of one leapfrog step, in launch order:
+ * inverse z, y, x passes; [W mu, W^T v]; forward x, y, z passes. */
+int dr_profile_kernels(dr_chain* c, double* ms, int max_k, int* n_k, int64_t* n_steps);
 
 /* --- kriging matrix build (kriging_matrix.h:147-343) --------------------- */
 typedef struct dr_csr dr_csr;

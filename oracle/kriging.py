@@ -7,7 +7,10 @@ flat-offset range check (kriging_matrix.h:263-266); a bounding-box index that
 falls outside the bounded index table (undefined behaviour in the reference,
 kriging_matrix.h:275-284) is skipped and counted in ``stats['oob']``.
 
-Weights are solved in float64 (the reference solves in float32 with Eigen's
+Kernel values in K and k' are float32 (as in the reference, which evaluates
+rbf<float>); they are computed here as the float32 rounding of the float64
+kernel, so they do not depend on a platform's float exp/pow ulps.  Weights
+are solved in float64 (the reference solves in float32 with Eigen's
 colPivHouseholderQr, kriging_matrix.h:309-313; those float32 bits are parity
 unpinned).  Pinned by: brute-force integer pattern enumeration and the
 FP64 residual ||K w - k'|| / ||k'|| (tests/test_oracle_kriging.py).
@@ -63,9 +66,9 @@ def neighborhood_perturbation(qform, radius):
 def perturbation_kernel_distances(P, qform, theta, dtype=np.float32):
     """K(i, j) = rbf(||x_i - x_j||), unit diagonal  (kriging_matrix.h:111-138)."""
     X = _mm(P, qform)
-    D = _norm3(X[:, None, :] - X[None, :, :]).astype(dtype)
-    t = [dtype(v) for v in theta]
-    K = spectrum.rbf(D, t[1], t[2], t[0]).astype(dtype)
+    D = _norm3(X[:, None, :] - X[None, :, :]).astype(np.float64)
+    t = [float(np.float32(v)) for v in theta]
+    K = spectrum.rbf(D, t[1], t[2], t[0]).astype(np.float32).astype(dtype)
     np.fill_diagonal(K, 1)
     return K
 
@@ -95,7 +98,7 @@ def kriging_pattern(hr, std, q_hr, q_std, theta, radius):
     Qh = np.asarray(q_hr, dtype=np.float32).reshape(4, 4)
     Qs = np.asarray(q_std, dtype=np.float32).reshape(4, 4)
     ijk_std = images.get_nonzero_indices(std)
-    tf = [np.float32(v) for v in theta]
+    tf = [float(np.float32(v)) for v in theta]
     eps0 = np.float32(1e-5)
     rows = []
     oob = 0
@@ -142,7 +145,7 @@ def kriging_pattern(hr, std, q_hr, q_std, theta, radius):
         for c in range(4):
             ss = (ss + (d[:, c] * d[:, c]).astype(np.float32)).astype(np.float32)
         dist = (np.sqrt(ss).astype(np.float32) + eps0).astype(np.float32)
-        kp = spectrum.rbf(dist, tf[1], tf[2], tf[0]).astype(np.float32)
+        kp = spectrum.rbf(dist.astype(np.float64), tf[1], tf[2], tf[0]).astype(np.float32)
         rows.append((col[sel], sel, kp))
     return rows, P, {"oob": oob, "n_rows": ijk_std.shape[0], "ncol": int((bidx >= 0).sum())}
 

@@ -169,3 +169,26 @@ def test_too_many_negative_eigenvalues(gpu_available):
     p = small_single()
     with pytest.raises(DrDomainError):
         engine_chain(p, (1.0, 0.0, 1.0))
+
+
+@pytest.mark.parametrize("geom", ["aligned", "oblique"])
+def test_kriging_build_matches_oracle(gpu_available, geom):
+    """Pattern (row pointers, columns) bit-exact; float64 weights within 1e-6."""
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
+    from oracle import kriging
+    theta = (0.806, 0.132, 1.5)
+    p = problem.dual_problem((14, 12, 10), 2.0, theta, seed_gp=9)
+    q_std = p["q_std"].copy()
+    if geom == "oblique":
+        q_std[0, 3] += 0.37  # stdres origin off the half-voxel grid
+        q_std[1, 3] -= 0.21
+    radius = 2.3 * 2.0
+    rp, cols, vals, nrow, ncol, st = kriging.get_sparse_kriging_matrix_data(
+        p["hr"], p["std"], p["q_hr"], q_std, theta, radius)
+    W, info = kriging_build(p["hr"], p["q_hr"], p["std"], q_std, theta, radius, return_info=True)
+    assert W.shape == (nrow, ncol)
+    assert np.array_equal(W.indptr, rp)
+    assert np.array_equal(W.indices, cols)
+    assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
+    assert info["n_unique_solves"] <= nrow
