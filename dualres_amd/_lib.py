@@ -71,6 +71,8 @@ SIGNATURES = {
     "dr_get_lambda": [_P, _D],
     "dr_set_data": [_P, _D, C.c_int64, _D, C.c_int64, _I64, _I32, _D],
     "dr_add_data_to_mu": [_P],
+    "dr_w_layout": [_P, C.c_int],
+    "dr_w_stats": [_P, _I64],
     "dr_set_output_indices": [_P, _I32, C.c_int64],
     "dr_get_mu_output": [_P, _D, C.c_int64],
     "dr_get_mu_data": [_P, _D, C.c_int64],

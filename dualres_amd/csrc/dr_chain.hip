@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "dr_kernels.hpp"
@@ -91,17 +92,6 @@ __global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a, const dou
   if (threadIdx.x == 0) sums[s] = nb > 0 ? r : 0.0;
 }
 
-// twiddle table exp(-2 pi i e / M), e in [0, M), rounded from long double
-template <typename T>
-static std::vector<cx<T>> twiddles(int M) {
-  std::vector<cx<T>> t(M);
-  const long double pi = 3.141592653589793238462643383279502884L;
-  for (int e = 0; e < M; ++e) {
-    const long double a = -2.0L * pi * (long double)e / (long double)M;
-    t[e] = mk<T>((T)std::cos(a), (T)std::sin(a));
-  }
-  return t;
-}
 
 // ---------------------------------------------------------------- reduction slots
 enum Slot {
@@ -131,6 +121,9 @@ struct ChainBase {
   long n_nonpos = 0;
   double max_eig = 0.0;
   bool data_set = false;
+  bool use_cls = false;    // class-stencil W kernels active
+  bool force_csr = false;  // keep the plain CSR kernels (tests)
+  int64_t wstats[5] = {0, 0, 0, 0, 0};
   // cached energies of the current state A
   bool stats_valid = false;
   double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
@@ -180,6 +173,9 @@ struct Chain : public ChainBase {
   DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
   DevBuf<T> Wvals, WTvals, Ysd, vsd, outbuf;
   DevBuf<double> part, sums;
+  // class-stencil W (see build_classes)
+  DevBuf<int> crg0, crcls, canchor, ccoff, ccn, ccdoff, cvcls, ctcoff, ctcn, ctcdoff;
+  DevBuf<T> ccw, ctcw, Vg;
   long pstride = 0;
   int slot_nblk[NSLOT] = {0};
   std::vector<int32_t> out_g;  // region grid index of each output voxel
@@ -233,7 +229,7 @@ struct Chain : public ChainBase {
   void xr2c(int nrows, Pro pro, Epi epi, int slot = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, nrows, kTileElems);
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileElems);
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -246,7 +242,7 @@ struct Chain : public ChainBase {
   void xc2r(int nrows, Pro pro, Epi epi, int slot = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, nrows, kTileElems);
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileElems);
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -255,6 +251,9 @@ struct Chain : public ChainBase {
     DR_HIP(hipGetLastError());
     mark();
   }
+  // lines per x-pass workgroup: enough work per group (>= ~1024 elements) but
+  // at least ~2048 workgroups when there are enough rows to fill 256 CUs
+  static int xlines(int nrows, int Mh) { return std::max(std::max(1, nrows / 2048), cdiv(1024, std::max(Mh, 1))); }
   void note_blocks(int slot, long nb) {
     DR_REQUIRE(nb <= pstride, DR_ELOGIC, "reduction grid exceeds partial buffer");
     slot_nblk[slot] = (int)nb;
@@ -298,12 +297,14 @@ struct Chain : public ChainBase {
     muA.alloc(Nh);
     muB.alloc(Nh);
     p.alloc(Nh);
+    p.zero(stream);
     T1.alloc(Nh);
     T2.alloc(Nh);
     if (compat()) {
       miA.alloc(Nh);
       miB.alloc(Nh);
       pim.alloc(Nh);
+      pim.zero(stream);
     }
     int r[3];
     for (int a = 0; a < 3; ++a) {
@@ -458,6 +459,122 @@ struct Chain : public ChainBase {
     stats_valid = false;
   }
 
+  // ------------------------------------------------------------ class-stencil W
+  // Rows equal up to translation share a class: (entry count, relative grid
+  // offsets, weights) hashed, then verified entry by entry.
+  static void classify(long nrows, const std::vector<int32_t>& rp, const std::vector<int32_t>& doff,
+                       const std::vector<T>& w, std::vector<int32_t>& cls, std::vector<int32_t>& coff,
+                       std::vector<int32_t>& cn, std::vector<int32_t>& cdoff, std::vector<T>& cw) {
+    struct Key {
+      uint64_t a, b;
+      int n;
+      bool operator==(const Key& o) const { return a == o.a && b == o.b && n == o.n; }
+    };
+    struct KH {
+      size_t operator()(const Key& k) const { return (size_t)(k.a ^ (k.b * 0x9E3779B97F4A7C15ULL) ^ (uint64_t)k.n); }
+    };
+    auto mix = [](uint64_t x) {
+      x ^= x >> 33; x *= 0xff51afd7ed558ccdULL; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL; x ^= x >> 33;
+      return x;
+    };
+    std::unordered_map<Key, std::vector<int32_t>, KH> seen;  // key -> candidate classes
+    std::vector<int32_t> crow;                                // representative row of each class
+    cls.assign(nrows, -1);
+    for (long i = 0; i < nrows; ++i) {
+      const int32_t a = rp[i], n = rp[i + 1] - rp[i];
+      if (n == 0) continue;
+      uint64_t h1 = 0, h2 = 0;
+      for (int32_t t = 0; t < n; ++t) {
+        uint64_t wb = 0;
+        std::memcpy(&wb, &w[a + t], sizeof(T));
+        const uint64_t k = ((uint64_t)(uint32_t)doff[a + t] << 32) ^ wb;
+        h1 += mix(k + 0x9E3779B97F4A7C15ULL * (uint64_t)(t + 1));
+        h2 += mix((k * 0xD6E8FEB86659FD93ULL) ^ ((uint64_t)(t + 7) * 0xA0761D6478BD642FULL));
+      }
+      auto& cand = seen[Key{h1, h2, n}];
+      int32_t found = -1;
+      for (int32_t c : cand) {
+        const int32_t r = crow[c], b = rp[r];
+        if (std::memcmp(&doff[a], &doff[b], n * sizeof(int32_t)) == 0 &&
+            std::memcmp(&w[a], &w[b], n * sizeof(T)) == 0) {
+          found = c;
+          break;
+        }
+      }
+      if (found < 0) {
+        found = (int32_t)crow.size();
+        crow.push_back((int32_t)i);
+        coff.push_back((int32_t)cdoff.size());
+        cn.push_back(n);
+        cdoff.insert(cdoff.end(), doff.begin() + a, doff.begin() + a + n);
+        cw.insert(cw.end(), w.begin() + a, w.begin() + a + n);
+        cand.push_back(found);
+      }
+      cls[i] = found;
+    }
+  }
+
+  template <typename V>
+  void upload(DevBuf<V>& d, const std::vector<V>& h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) DR_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, stream));
+  }
+
+  // cols: image-region grid index of every entry; tr*: transposed CSR.
+  bool build_classes(const std::vector<int32_t>& rp, const std::vector<int32_t>& cols, const std::vector<T>& vals,
+                     const std::vector<int32_t>& trp, const std::vector<int32_t>& trows,
+                     const std::vector<T>& tvals, const std::vector<int32_t>& g) {
+    const long nnz = rp[n_s];
+    // anchors: a distinct grid position per row, taken from its own entries
+    std::vector<uint8_t> claimed(nreg, 0);
+    std::vector<int32_t> anchor(n_s, 0), rg0(n_s, 0);
+    for (long i = 0; i < n_s; ++i) {
+      if (rp[i + 1] == rp[i]) continue;
+      rg0[i] = cols[rp[i]];
+      int32_t an = -1;
+      for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
+        if (!claimed[cols[t]]) {
+          an = cols[t];
+          break;
+        }
+      if (an < 0) return false;  // no free anchor: keep the CSR kernels
+      claimed[an] = 1;
+      anchor[i] = an;
+    }
+    std::vector<int32_t> doff(nnz);
+    for (long i = 0; i < n_s; ++i)
+      for (int32_t t = rp[i]; t < rp[i + 1]; ++t) doff[t] = cols[t] - rg0[i];
+    std::vector<int32_t> rcls, coff, cn, cdoff;
+    std::vector<T> cw;
+    classify(n_s, rp, doff, vals, rcls, coff, cn, cdoff, cw);
+    for (long d = 0; d < n_h; ++d)
+      for (int32_t t = trp[d]; t < trp[d + 1]; ++t) doff[t] = g[d] - anchor[trows[t]];
+    std::vector<int32_t> vcls, tcoff, tcn, tcdoff;
+    std::vector<T> tcw;
+    classify(n_h, trp, doff, tvals, vcls, tcoff, tcn, tcdoff, tcw);
+    upload(crg0, rg0);
+    upload(crcls, rcls);
+    upload(canchor, anchor);
+    upload(ccoff, coff);
+    upload(ccn, cn);
+    upload(ccdoff, cdoff);
+    upload(ccw, cw);
+    upload(cvcls, vcls);
+    upload(ctcoff, tcoff);
+    upload(ctcn, tcn);
+    upload(ctcdoff, tcdoff);
+    upload(ctcw, tcw);
+    Vg.alloc(nreg);
+    Vg.zero(stream);
+    DR_HIP(hipStreamSynchronize(stream));
+    wstats[0] = (int64_t)cn.size();
+    wstats[1] = (int64_t)cdoff.size();
+    wstats[2] = (int64_t)tcn.size();
+    wstats[3] = (int64_t)tcdoff.size();
+    wstats[4] = nnz;
+    return true;
+  }
+
   void upload_data() override {
     // data on the region grid
     std::vector<T> yg(nreg, (T)0);
@@ -500,27 +617,31 @@ struct Chain : public ChainBase {
           tvals[fill[c]] = (T)W_vals[j];
           fill[c]++;
         }
-      Wrp.alloc(n_s + 1);
-      Wcols.alloc(std::max<long>(nnz, 1));
-      Wvals.alloc(std::max<long>(nnz, 1));
-      WTrp.alloc(n_h + 1);
-      WTrows.alloc(std::max<long>(nnz, 1));
-      WTvals.alloc(std::max<long>(nnz, 1));
       Ysd.alloc(n_s);
       vsd.alloc(n_s);
       WTv.alloc(nreg);
       WTv.zero(stream);
       std::vector<T> ys(n_s);
       for (long i = 0; i < n_s; ++i) ys[i] = (T)Ys[i];
-      DR_HIP(hipMemcpyAsync(Wrp.p, rp.data(), (n_s + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-      if (nnz) {
-        DR_HIP(hipMemcpyAsync(Wcols.p, cols.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        DR_HIP(hipMemcpyAsync(Wvals.p, vals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
-        DR_HIP(hipMemcpyAsync(WTrows.p, trows.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        DR_HIP(hipMemcpyAsync(WTvals.p, tvals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
-      }
-      DR_HIP(hipMemcpyAsync(WTrp.p, trp.data(), (n_h + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
       DR_HIP(hipMemcpyAsync(Ysd.p, ys.data(), n_s * sizeof(T), hipMemcpyHostToDevice, stream));
+      use_cls = !force_csr && build_classes(rp, cols, vals, trp, trows, tvals, g);
+      if (!use_cls) {
+        Wrp.alloc(n_s + 1);
+        Wcols.alloc(std::max<long>(nnz, 1));
+        Wvals.alloc(std::max<long>(nnz, 1));
+        WTrp.alloc(n_h + 1);
+        WTrows.alloc(std::max<long>(nnz, 1));
+        WTvals.alloc(std::max<long>(nnz, 1));
+        DR_HIP(hipMemcpyAsync(Wrp.p, rp.data(), (n_s + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        if (nnz) {
+          DR_HIP(hipMemcpyAsync(Wcols.p, cols.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+          DR_HIP(hipMemcpyAsync(Wvals.p, vals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
+          DR_HIP(hipMemcpyAsync(WTrows.p, trows.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+          DR_HIP(hipMemcpyAsync(WTvals.p, tvals.data(), nnz * sizeof(T), hipMemcpyHostToDevice, stream));
+        }
+        DR_HIP(hipMemcpyAsync(WTrp.p, trp.data(), (n_h + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        DR_HIP(hipStreamSynchronize(stream));
+      }
     }
     DR_HIP(hipStreamSynchronize(stream));
     data_set = true;
@@ -555,11 +676,23 @@ struct Chain : public ChainBase {
     if (cfg.n_datasets != 2) return;
     const int g1 = cdiv(n_s, kThreads / 64);
     if (slot >= 0) note_blocks(slot, g1);
+    const int g2 = cdiv(n_h, kThreads / 16);
+    if (use_cls) {
+      hipLaunchKernelGGL(k_wmu_cls<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, crg0.p, crcls.p,
+                         WClasses{ccoff.p, ccn.p, ccdoff.p}, ccw.p, R, Ysd.p, (T)sigma[1], canchor.p, Vg.p,
+                         slot >= 0 ? pslot(slot) : nullptr);
+      DR_HIP(hipGetLastError());
+      mark();
+      hipLaunchKernelGGL(k_wtv_cls<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, gI.p, cvcls.p,
+                         WClasses{ctcoff.p, ctcn.p, ctcdoff.p}, ctcw.p, Vg.p, WTv.p);
+      DR_HIP(hipGetLastError());
+      mark();
+      return;
+    }
     hipLaunchKernelGGL(k_wmu<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, Wrp.p, Wcols.p, Wvals.p, R, Ysd.p,
                        (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr);
     DR_HIP(hipGetLastError());
     mark();
-    const int g2 = cdiv(n_h, kThreads / 16);
     hipLaunchKernelGGL(k_wtv<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, WTrp.p, WTrows.p, WTvals.p, vsd.p,
                        gI.p, WTv.p);
     DR_HIP(hipGetLastError());
@@ -569,52 +702,87 @@ struct Chain : public ChainBase {
     return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr, (T)sigma[0], reg[0],
                            slot >= 0 ? pslot(slot) : nullptr, 0.0};
   }
-  // gradient at the state held in X (real space result R), then p += coef * g
-  void grad_step(const cx<T>* X, const cx<T>* Xi, T* R, T coef, bool red, int s_sh, int s_ss, int s_prior,
-                 int s_priori, int s_kin, int s_kini) {
-    w_products(R, red ? s_ss : -1);
-    EpiMomentum<T> em{X, p.p, invlam.p, (long)M[1] * H0, coef, (T)sigma[0], H0, Xi,
-                      compat() ? pim.p : nullptr, red ? 1 : 0, part.p, pstride, s_prior, s_priori, s_kin, s_kini,
-                      0.0, 0.0, 0.0, 0.0};
-    if (red) {
-      // the epilogue writes four slots from the same grid
-      const int NB = M[1] * H0;
-      const int tbs = pick_tbs(M[2], M[2], sizeof(cx<T>), false, NB, kTileElems);
-      const long nb = cdiv(NB, 1 << tbs);
-      note_blocks(s_prior, nb);
-      note_blocks(s_priori, nb);
-      note_blocks(s_kin, nb);
-      note_blocks(s_kini, nb);
-    }
-    r2c_region(residual(R, red ? s_sh : -1), em, red ? s_sh : -1, -1);
+  // inverse y and x passes from T1 (z < n2) to the real region field R
+  void c2r_yx(T* R) {
+    strided<T, true>(1, reg[2], H0, M[1], reg[1], LoadS<T>{{}, T1.p, (long)M[1] * H0, H0},
+                     StoreS<T>{{}, T2.p, (long)reg[1] * H0, H0});
+    xc2r<T>(reg[1] * reg[2], LoadRowC<T>{{}, T2.p, H0}, StoreRowR<T>{{}, R, reg[0], (T)(1.0 / (double)N)});
+  }
+  // forward x and y passes of the scattered residual into T1 (z < n2)
+  void r2c_xy(LoadResidual<T> pro, int pro_slot) {
+    xr2c<T>(reg[1] * reg[2], pro, StoreRowC<T>{{}, T2.p, H0}, pro_slot);
+    strided<T, false>(1, reg[2], H0, reg[1], M[1], LoadS<T>{{}, T2.p, (long)reg[1] * H0, H0},
+                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
+  }
+  // forward z pass, kick + drift, inverse z pass, in place on T1
+  void zfused(MidLeapfrog<T> mid, int mid_slot) {
+    const AxisFFT<T>& ad = axis_dev<T>(2);
+    const int NB = M[1] * H0;
+    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, NB, kTileElems);
+    const int TB = 1 << tbs;
+    const int grid = cdiv(NB, TB);
+    const size_t lds = (size_t)ad.M * TB * sizeof(cx<T>);
+    if (mid_slot >= 0) note_blocks(mid_slot, grid);
+    const long sk = (long)M[1] * H0;
+    hipLaunchKernelGGL((k_strided2<T, LoadS<T>, MidLeapfrog<T>, StoreS<T>>), dim3(grid), dim3(kThreads), lds, stream,
+                       ad, NB, reg[2], reg[2], tbs, LoadS<T>{{}, T1.p, 0, sk}, mid, StoreS<T>{{}, T1.p, 0, sk});
+    DR_HIP(hipGetLastError());
+    mark();
+  }
+  int zgrid() const {
+    const int NB = M[1] * H0;
+    return cdiv(NB, 1 << pick_tbs(M[2], M[2], sizeof(cx<T>), false, NB, kTileElems));
   }
 
   // ------------------------------------------------------------ HMC iteration
-  void leapfrog_core(double eps, int L, bool start_red) {
+  // Leapfrog from (muA, p) to (muB, p'), MultiResParameters.h:690-700:
+  //   p += eps/2 g(mu); L times { mu += eps M^-1 p; p += k eps g(mu) }.
+  // Each gradient is inverse z,y,x -> W products -> forward x,y,z; the
+  // forward z pass of one gradient, the kick, the drift and the inverse z
+  // pass of the next gradient run as one kernel.  Reductions give the prior
+  // energy at A and B, the kinetic energy at the end and both SSEs.
+  void leapfrog_core(double eps, int L) {
     const long sk = (long)M[1] * H0;
-    // gradient at A and half step
     c2r_region(LoadS<T>{{}, muA.p, 0, sk}, RA.p);
-    grad_step(muA.p, compat() ? miA.p : nullptr, RA.p, (T)(0.5 * eps), start_red, S_SSEH_A, S_SSES_A, S_PRIOR_A,
-              S_PRIORI_A, S_TMP, S_TMP);
+    w_products(RA.p, S_SSES_A);
+    r2c_xy(residual(RA.p, S_SSEH_A), S_SSEH_A);
     for (int s = 0; s < L; ++s) {
-      const double k = (s == L - 1) ? 0.5 : 1.0;
-      const cx<T>* src = s == 0 ? muA.p : muB.p;
-      const cx<T>* srci = compat() ? (s == 0 ? miA.p : miB.p) : nullptr;
-      LoadPosUpdate<T> pu{{}, src, muB.p, p.p, invlam.p, sk, (T)eps, (T)sigma[0], srci,
-                          compat() ? miB.p : nullptr, compat() ? pim.p : nullptr};
+      const bool last = s == L - 1;
       if (prof && L >= 2 && s == 1) DR_HIP(hipEventRecord(ev[0], stream));
       if (prof && L >= 3 && s == 1) {
         rec = 0;
         mark();
       }
-      c2r_region(pu, RB.p);
-      grad_step(muB.p, compat() ? miB.p : nullptr, RB.p, (T)(k * eps), s == L - 1, S_SSEH_B, S_SSES_B, S_PRIOR_B,
-                S_PRIORI_B, S_KIN_B, S_KINI_B);
+      MidLeapfrog<T> mid{s == 0 ? muA.p : muB.p, muB.p, p.p, invlam.p, sk, (T)((s == 0 ? 0.5 : 1.0) * eps),
+                         (T)eps, (T)sigma[0], H0, s == 0 ? 1 : 0, pslot(S_PRIOR_A), 0.0};
+      zfused(mid, s == 0 ? S_PRIOR_A : -1);
+      c2r_yx(RB.p);
+      w_products(RB.p, last ? S_SSES_B : -1);
+      r2c_xy(residual(RB.p, last ? S_SSEH_B : -1), last ? S_SSEH_B : -1);
       if (rec >= 0) {
         nrec = rec;
         rec = -1;
       }
       if (prof && L >= 2 && s == L - 1) DR_HIP(hipEventRecord(ev[1], stream));
+    }
+    // final half kick at B with the energies of the end point
+    EpiMomentum<T> em{muB.p, p.p, invlam.p, sk, (T)(0.5 * eps), (T)sigma[0], H0, nullptr, nullptr, 1, part.p,
+                      pstride, S_PRIOR_B, S_TMP, S_KIN_B, S_TMP, 0.0, 0.0, 0.0, 0.0};
+    const int zb = zgrid();
+    note_blocks(S_PRIOR_B, zb);
+    note_blocks(S_KIN_B, zb);
+    note_blocks(S_TMP, zb);
+    strided<T, false>(2, 1, M[1] * H0, reg[2], M[2], LoadS<T>{{}, T1.p, 0, sk}, em);
+    if (compat()) {
+      // imaginary chain: all L steps per mode in one pass
+      const int g = (int)std::min<long>(cdiv(Nh, kThreads), 4096);
+      note_blocks(S_PRIORI_A, g);
+      note_blocks(S_PRIORI_B, g);
+      note_blocks(S_KINI_B, g);
+      hipLaunchKernelGGL(k_imag_leapfrog<T>, dim3(g), dim3(kThreads), 0, stream, Nh, H0, miA.p, miB.p, pim.p,
+                         invlam.p, (T)eps, (T)(0.5 * eps), (T)eps, (T)sigma[0], L, pslot(S_PRIORI_A),
+                         pslot(S_PRIORI_B), pslot(S_KINI_B));
+      DR_HIP(hipGetLastError());
     }
   }
 
@@ -637,7 +805,7 @@ struct Chain : public ChainBase {
       if (zr) upload_real(zi, zbuf);
       noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I, EpiNoiseMomentum<T>{{}, pim.p, invlam.p, sk, (T)sigma[0]});
     }
-    leapfrog_core(eps, L, true);
+    leapfrog_core(eps, L);
     finalize();
     double h[NSLOT];
     read_sums(h);
@@ -688,7 +856,7 @@ struct Chain : public ChainBase {
     clear_slots();
     quad(p.p, S_KIN_A, (T)sigma[0]);
     if (compat()) quad(pim.p, S_KINI_A, (T)sigma[0]);
-    leapfrog_core(eps, L, true);
+    leapfrog_core(eps, L);
     finalize();
     double h[NSLOT];
     read_sums(h);
@@ -930,6 +1098,18 @@ int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, in
       k.W_vals.assign(W_vals, W_vals + nnz);
     }
     k.upload_data();
+  });
+}
+
+int dr_w_layout(dr_chain* c, int force_csr) {
+  return guarded([&] { c->impl->force_csr = force_csr != 0; });
+}
+
+int dr_w_stats(dr_chain* c, int64_t* stats) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    stats[0] = k.use_cls ? 1 : 0;
+    for (int i = 0; i < 5; ++i) stats[1 + i] = k.wstats[i];
   });
 }
 

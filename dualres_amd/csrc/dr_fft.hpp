@@ -21,6 +21,7 @@
 // the passes to the image region of the embedding torus.
 #pragma once
 
+#include <cmath>
 #include <vector>
 
 #include "dr_common.hpp"
@@ -240,6 +241,43 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
   epi.finish(red, blk);
 }
 
+// Forward transform, pointwise update, inverse transform of the same tile:
+// the leapfrog kick and drift in the spectral domain sit between the forward
+// z pass of one gradient and the inverse z pass of the next.
+//   cx<T> Mid::apply(int k, int b, cx<T> v)  for every k < M.
+template <typename T, class Pro, class Mid, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid, Epi epi) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[kThreads];
+  const int TB = 1 << tbs;
+  const int b0 = blockIdx.x * TB;
+  const int M = ax.M;
+  for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    cx<T> v = mk<T>(0, 0);
+    if (k < k_in && b0 + b < NB) v = pro.load(0, k, b0 + b);
+    lds[k * TB + b] = v;
+  }
+  __syncthreads();
+  fft_lds<T, false>(lds, TB, tbs, ax);
+  for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    if (b0 + b < NB) lds[k * TB + b] = mid.apply(k, b0 + b, lds[k * TB + b]);
+  }
+  __syncthreads();
+  fft_lds<T, true>(lds, TB, tbs, ax);
+  for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    if (b0 + b < NB) epi.store(0, k, b0 + b, lds[k * TB + b]);
+  }
+  const int blk = blockIdx.x;
+  pro.finish(red, blk);
+  mid.finish(red, blk);
+  epi.finish(red, blk);
+}
+
 // Real rows of length M0 (first n_in samples nonzero) -> M0/2+1 complex.
 // ax describes the complex FFT of length Mh = M0/2 with tws = 2 into the
 // length-M0 table, so ax.tw[k] = exp(-2 pi i k / M0).
@@ -332,6 +370,18 @@ k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
 
 // ---------------------------------------------------------------- host plan helpers
 std::vector<int> factorize(int M);
+
+// twiddle table exp(-2 pi i e / M), e in [0, M), rounded from long double
+template <typename T>
+std::vector<cx<T>> twiddles(int M) {
+  std::vector<cx<T>> t(M);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int e = 0; e < M; ++e) {
+    const long double a = -2.0L * pi * (long double)e / (long double)M;
+    t[e] = mk<T>((T)std::cos(a), (T)std::sin(a));
+  }
+  return t;
+}
 
 template <typename T>
 AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws) {

@@ -89,6 +89,66 @@ template <typename T> struct EpiMomentum {
   }
 };
 
+// kick then drift between two gradients (MultiResParameters.h:690-699):
+//   p  += coef (F(S r) - X/lambda)      with the gradient taken at X
+//   X' =  X + eps p / lambda_m          written to Xd and returned for the
+//                                       inverse pass of the next gradient.
+// Optionally reduces the prior energy of X.
+template <typename T> struct MidLeapfrog {
+  const cx<T>* X; cx<T>* Xd; cx<T>* p; const T* invlam; long sk; T coef; T eps; T sig; int H0;
+  int do_red; double* part; double a0;
+  __device__ __forceinline__ cx<T> apply(int k, int b, cx<T> v) {
+    const long i = (long)k * sk + b;
+    const T il = invlam[i];
+    const cx<T> x = X[i];
+    const cx<T> pn = p[i] + (v - x * il) * coef;
+    p[i] = pn;
+    const T c = il > (T)0 ? eps / (il + sig) : (T)0;
+    const cx<T> m = x + pn * c;
+    Xd[i] = m;
+    if (do_red) a0 += herm_w(b % H0, H0) * (double)il * ((double)x.x * x.x + (double)x.y * x.y);
+    return m;
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (do_red) red_write(a0, red, part, blk);
+  }
+};
+
+// The data-free imaginary chain of the reference's complex state evolves under
+// the prior alone, so every Fourier mode is an independent linear oscillator:
+// all L leapfrog steps run in registers in one pass per iteration, with the
+// same per-step arithmetic as the real chain.  Reduces its prior energy before
+// and after and its final kinetic energy.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_imag_leapfrog(long Nh, int H0, const cx<T>* __restrict__ muA, cx<T>* __restrict__ muB, cx<T>* __restrict__ p,
+                const T* __restrict__ invlam, T eps, T half_coef, T full_coef, T sig, int L, double* partA,
+                double* partB, double* partK) {
+  __shared__ double red[kThreads];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < Nh; i += (long)gridDim.x * kThreads) {
+    const T il = invlam[i];
+    const T c = il > (T)0 ? eps / (il + sig) : (T)0;
+    cx<T> m = muA[i];
+    cx<T> q = p[i];
+    const double w = herm_w((int)(i % H0), H0);
+    a0 += w * (double)il * ((double)m.x * m.x + (double)m.y * m.y);
+    q = q - m * (il * half_coef);
+    for (int s = 0; s < L; ++s) {
+      m = m + q * c;
+      q = q - m * (il * (s == L - 1 ? half_coef : full_coef));
+    }
+    muB[i] = m;
+    p[i] = q;
+    const double ilm = il > (T)0 ? 1.0 / (double)(il + sig) : 0.0;
+    a1 += w * (double)il * ((double)m.x * m.x + (double)m.y * m.y);
+    a2 += w * ilm * ((double)q.x * q.x + (double)q.y * q.y);
+  }
+  red_write(a0, red, partA, blockIdx.x);
+  red_write(a1, red, partB, blockIdx.x);
+  red_write(a2, red, partK, blockIdx.x);
+}
+
 // gradient in the spectral domain: G = F(S r) - X/lambda
 template <typename T> struct EpiGrad : NoRed {
   const cx<T>* X; cx<T>* G; const T* invlam; long sk;
@@ -292,6 +352,65 @@ k_wtv(int n_h, const int* __restrict__ rp, const int* __restrict__ rows, const T
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
   if (d < n_h && lane == 0) out[gidx[d]] = acc;
+}
+
+// Class-stencil form of the same two products.  Rows of W that are equal up
+// to a translation on the image-region grid (same relative offsets, same
+// weights) share one class; a row stores only its anchor grid index and class.
+// Interior kriging rows all fall in one class, so their weights and offsets
+// stay in cache and only the boundary rows stream explicit entries.  Entry
+// order within a row is the CSR order, so sums match k_wmu / k_wtv bitwise.
+struct WClasses {
+  const int* off;   // first entry of each class
+  const int* n;     // entries per class
+  const int* doff;  // grid offset of each entry relative to the anchor
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wmu_cls(int n_s, const int* __restrict__ rg0, const int* __restrict__ rcls, WClasses C, const T* __restrict__ w,
+          const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, const int* __restrict__ anchor,
+          T* __restrict__ V, double* part) {
+  __shared__ double red[kThreads];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  double e2 = 0.0;
+  if (row < n_s) {
+    const int c = rcls[row];
+    T acc = (T)0;
+    if (c >= 0) {
+      const int base = rg0[row], o = C.off[c], n = C.n[c];
+      for (int j = lane; j < n; j += 64) acc += w[o + j] * R[base + C.doff[o + j]];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const T e = Ys[row] - acc;
+      if (c >= 0) V[anchor[row]] = sig_s * e;
+      e2 = (double)e * (double)e;
+    }
+  }
+  if (part) red_write(e2, red, part, blockIdx.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wtv_cls(int n_h, const int* __restrict__ vg, const int* __restrict__ vcls, WClasses C, const T* __restrict__ w,
+          const T* __restrict__ V, T* __restrict__ out) {
+  const int lane = threadIdx.x & 15;
+  const int d = blockIdx.x * (kThreads / 16) + (threadIdx.x >> 4);
+  T acc = (T)0;
+  int g = 0;
+  if (d < n_h) {
+    const int c = vcls[d];
+    g = vg[d];
+    if (c >= 0) {
+      const int o = C.off[c], n = C.n[c];
+      for (int j = lane; j < n; j += 16) acc += w[o + j] * V[g - C.doff[o + j]];
+    }
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+  if (d < n_h && lane == 0) out[g] = acc;
 }
 
 // sum over the half spectrum of w |X|^2 a(k) with a = invlam (prior) or

@@ -101,6 +101,15 @@ int dr_get_lambda(dr_chain* c, double* lambda_out);
 int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, int64_t n_s,
                 const int64_t* W_rowptr, const int32_t* W_cols, const double* W_vals);
 
+/* Device layout of W.  By default rows of W (and of W^T) that are equal up
+ * to a translation on the image grid share one stored class (same offsets,
+ * same weights, same summation order as CSR); force_csr = 1 keeps plain CSR.
+ * Call before dr_set_data.  No counterpart in the reference. */
+int dr_w_layout(dr_chain* c, int force_csr);
+/* stats[0] = class layout active; [1] row classes; [2] row-class entries;
+ * [3] voxel (W^T) classes; [4] voxel-class entries; [5] nnz(W). */
+int dr_w_stats(dr_chain* c, int64_t* stats);
+
 /* MultiResParameters::add_data_to_mu (:631-641). */
 int dr_add_data_to_mu(dr_chain* c);
 /* MultiResParameters::set_output_indices (:646-667). */

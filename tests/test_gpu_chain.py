@@ -192,3 +192,25 @@ def test_kriging_build_matches_oracle(gpu_available, geom):
     assert np.array_equal(W.indices, cols)
     assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
     assert info["n_unique_solves"] <= nrow
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_w_class_layout_bitwise_equals_csr(gpu_available, precision):
+    """The class-stencil W kernels keep the CSR summation order: identical bits."""
+    from dualres_amd import Chain
+    p = small_dual(dims=(16, 14, 12))
+    outs = []
+    for force_csr in (True, False):
+        c = Chain(p["ijk"], THETA2, p["q_hr"], n_datasets=2, precision=precision, state_mode="real", seed=4)
+        c.w_layout(force_csr)
+        c.set_data(p["Yh"], p["Ys"], p["W"])
+        c.set_sigma_sq_inv([1.1, 0.6])
+        st = c.w_stats()
+        assert st["class_layout"] == (not force_csr)
+        rng = np.random.default_rng(5)
+        z = (rng.standard_normal(c.N), rng.standard_normal(c.N))
+        outs.append((c.gradient(), c.update(0.05, 3, 0.5, noise=z)))
+        c.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1]["H0"] == outs[1][1]["H0"]
+    assert outs[0][1]["H1"] == outs[1][1]["H1"]

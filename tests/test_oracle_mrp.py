@@ -1,0 +1,142 @@
+"""Pins oracle/mrp.py (MultiResParameters restatement) by dense algebra,
+finite differences and leapfrog invariants on tiny tori."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from dualres_amd import problem
+from oracle import mrp, spectrum
+
+THETA = (0.806, 1.0, 1.5)  # all eigenvalues positive on the 8x4x4 torus
+
+
+def _tiny(dual=False, seed=0):
+    rng = np.random.default_rng(seed)
+    dims = (4, 3, 3)
+    ijk = np.array([[i, j, k] for k in range(3) for j in range(3) for i in range(4) if (i + j + k) % 3 != 1])
+    q = problem.qform_axis(1.0, dims)
+    o = mrp.MultiResParameters(2 if dual else 1, THETA, ijk, q, rng=mrp.NumpyRng(seed))
+    Yh = rng.standard_normal(ijk.shape[0])
+    if dual:
+        W = sp.random(5, ijk.shape[0], density=0.4, random_state=seed, format="csr")
+        data = mrp.MultiResData(Yh, rng.standard_normal(5), W)
+    else:
+        data = mrp.MultiResData(Yh)
+    o.sigma_sq_inv = [1.7, 0.6][: o.n_datasets]
+    z = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    o.mu = z
+    return o, data
+
+
+def _dense_C(o):
+    M0, M1, M2 = o.M
+    base = spectrum.torus_distance(o.M, np.asarray(problem.qform_axis(1.0, (4, 3, 3)))).astype(np.float64)
+    c = spectrum.rbf(base, THETA[1], THETA[2], THETA[0]).ravel()
+    idx = np.arange(o.N)
+    x, y, z = idx % M0, (idx // M0) % M1, idx // (M0 * M1)
+    dx = (x[:, None] - x[None, :]) % M0
+    dy = (y[:, None] - y[None, :]) % M1
+    dz = (z[:, None] - z[None, :]) % M2
+    return c[dx + M0 * (dy + M1 * dz)]
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_gradient_dense_circulant(dual):
+    o, data = _tiny(dual)
+    assert o.Z.size == 0
+    C = _dense_C(o)
+    mu = o.mu.reshape(-1)
+    r = o.sigma_sq_inv[0] * (data.Yh - mu[o.I].real)
+    if dual:
+        r = r + o.sigma_sq_inv[1] * (data.W.T @ (data.Ys - data.W @ mu[o.I].real))
+    g_dense = -np.linalg.solve(C, mu)
+    g_dense[o.I] += r
+    g = o.compute_gradient(data, o.mu).reshape(-1)
+    assert np.max(np.abs(g - g_dense)) < 1e-10 * np.max(np.abs(g_dense))
+
+
+def test_log_prior_dense():
+    o, data = _tiny()
+    C = _dense_C(o)
+    mu = o.mu.reshape(-1)
+    lp = -0.5 * (mu.real @ np.linalg.solve(C, mu.real) + mu.imag @ np.linalg.solve(C, mu.imag))
+    assert np.isclose(o._log_prior(o.mu), lp, rtol=1e-11)
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_gradient_finite_differences(dual):
+    o, data = _tiny(dual, seed=3)
+    g = o.compute_gradient(data, o.mu).reshape(-1)
+    h = 1e-5
+    for j in [0, 5, 17, o.I[3], o.I[-1]]:
+        e = np.zeros(o.N, dtype=complex)
+        e[j] = h
+        up = o._log_posterior(data, o.mu + e.reshape(o.shape))
+        dn = o._log_posterior(data, o.mu - e.reshape(o.shape))
+        assert np.isclose((up - dn) / (2 * h), g[j].real, rtol=1e-6, atol=1e-8)
+        ei = 1j * e
+        up = o._log_posterior(data, o.mu + ei.reshape(o.shape))
+        dn = o._log_posterior(data, o.mu - ei.reshape(o.shape))
+        assert np.isclose((up - dn) / (2 * h), g[j].imag, rtol=1e-6, atol=1e-8)
+
+
+def test_leapfrog_reversible_and_second_order():
+    o, data = _tiny(True, seed=5)
+    o._compute_mass_matrix_eigen_values()
+    rng = np.random.default_rng(1)
+    p0 = mrp.F(rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
+    mu0 = o.mu.copy()
+    mu1, p1 = o.leapfrog(data, mu0.copy(), p0.copy(), 0.05, 7)
+    mu2, p2 = o.leapfrog(data, mu1.copy(), -p1, 0.05, 7)
+    assert np.max(np.abs(mu2 - mu0)) < 1e-10 * np.max(np.abs(mu0))
+    assert np.max(np.abs(-p2 - p0)) < 1e-9 * np.max(np.abs(p0))
+
+    def dH(eps):
+        L = int(round(0.4 / eps))
+        m, p = o.leapfrog(data, mu0.copy(), p0.copy(), eps, L)
+        return (o._log_posterior(data, m) + o._potential_energy(p)) - (o._log_posterior(data, mu0) + o._potential_energy(p0))
+
+    r = dH(0.02) / dH(0.01)
+    assert 3.0 < r < 5.0
+
+
+def test_leapfrog_harmonic_closed_form():
+    """Every torus point observed, no W: mass matrix = posterior precision, so
+    each mode of mu - mu* follows the unit-frequency leapfrog map."""
+    o, _ = _tiny(False, seed=7)
+    o.I = np.arange(o.N)
+    rng = np.random.default_rng(2)
+    Y = rng.standard_normal(o.N)
+    data = mrp.MultiResData(Y)
+    o._compute_mass_matrix_eigen_values()
+    lamm = o.lam_mass.reshape(-1)
+    prec = (1.0 / o.lam.reshape(-1) + o.sigma_sq_inv[0])
+    assert np.allclose(lamm, prec)
+    # posterior mean mu* = M^-1 sigma Y (spectral solve)
+    mstar = mrp.B(mrp.F(o.sigma_sq_inv[0] * Y.reshape(o.shape)) / prec.reshape(o.shape)) / o.N
+    u0 = (rng.standard_normal(o.N)).reshape(o.shape)
+    # a real momentum: the data term only drives the real part of the state
+    p0 = rng.standard_normal(o.shape) + 0j
+    eps, L = 0.3, 5
+    mu1, p1 = o.leapfrog(data, (mstar + u0).astype(complex), p0.copy(), eps, L)
+    # w = M^-1 p in real space
+    w0 = mrp.F(mrp.B(p0) / (o.lam_mass * o.N))
+    w1 = mrp.F(mrp.B(p1) / (o.lam_mass * o.N))
+    A = np.array([[1 - eps ** 2 / 2, eps], [-eps * (1 - eps ** 2 / 4), 1 - eps ** 2 / 2]])
+    AL = np.linalg.matrix_power(A, L)
+    u_pred = AL[0, 0] * u0 + AL[0, 1] * w0
+    w_pred = AL[1, 0] * u0 + AL[1, 1] * w0
+    assert np.max(np.abs((mu1 - mstar) - u_pred)) < 1e-9
+    assert np.max(np.abs(w1 - w_pred)) < 1e-9
+
+
+def test_update_accept_and_energies_consistent():
+    o, data = _tiny(True, seed=9)
+    rng = np.random.default_rng(4)
+    zr, zi = rng.standard_normal(o.N), rng.standard_normal(o.N)
+    o.rng = mrp.ScriptedRng(normals=[(zr, zi)], uniforms=[0.0], gammas=[2.0, 3.0])
+    mh = o.update(data, 0.02, 3)
+    assert 0.0 <= mh <= 1.0
+    # E0 = -1/2 sum |z|^2 is the kinetic energy of the freshly drawn momentum
+    assert o.accepted  # u = 0 < R always unless R = 0
+    assert o.sigma_sq_inv == [2.0, 3.0]
