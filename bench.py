@@ -202,7 +202,7 @@ def main():
     ach = byt / (ks[dom] * 1e-3) / 1e9
     step_ms = sum(ks.values())
     log("[bench] kernels (ms/step): " + ", ".join(f"{k}={v:.4f}" for k, v in ks.items()))
-    fft_names = [k for k in ks if k.startswith(("c2r", "r2c"))]
+    fft_names = [k for k in ks if k.startswith(("c2r", "r2c", "z_fft"))]
     fft_bytes = sum(kernel_bytes(k, p, sb, main_chain) for k in fft_names)
     fft_ms = sum(ks[k] for k in fft_names)
     wst = main_chain.w_stats()
