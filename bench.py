@@ -97,9 +97,9 @@ def run_chain(c, p, steps, warmup, seed=8675309):
 def kernel_bytes(name, p, prec_bytes, c):
     """Algorithmic HBM bytes of one launch of the named kernel (read once + write once).
 
-    With the class-stencil W layout the W kernels read each stored class entry
-    (offset + weight) once, per-row anchors/classes, and the image-region
-    fields they gather from once."""
+    With the stencil + boundary W layout the W kernels read the shared stencil
+    once, per-row / per-voxel index words, each packed boundary entry (2-byte
+    offset + weight) once, and the image-region fields they gather from once."""
     s = prec_bytes
     cs = 2 * s
     M0, M1, M2 = EMBED
@@ -122,11 +122,17 @@ def kernel_bytes(name, p, prec_bytes, c):
     if name == "r2c_x_residual":
         return nreg * (3 * s + 1) + T2 * cs             # mu, Yh, W^T v, mask -> T2
     ws = c.w_stats()
-    if ws["class_layout"]:
-        if name == "W_mu":
-            return ws["row_class_entries"] * (4 + s) + n_s * (3 * 4 + 2 * s) + n_h * s
-        if name == "Wt_v":
-            return ws["voxel_class_entries"] * (4 + s) + n_h * (2 * 4 + s) + n_s * s
+    if ws["stencil_layout"]:
+        ni, nd, nb, eb = ws["interior_rows"], ws["stencil_entries"], ws["boundary_rows"], ws["boundary_entries"]
+        nt, nv = ws["touched_voxels"], ws["padded_voxels"]
+        if name == "W_mu_int":   # row/base/lattice words, Ys, v, the stencil, the region field R
+            return ni * (12 + 2 * s) + nd * (4 + s) + n_h * s
+        if name == "W_mu_bnd":   # row words, Ys, v, packed entries, R at the touched voxels
+            return nb * (16 + 2 * s) + eb * (2 + s) + nt * s
+        if name == "Wt_v_int":   # voxel slot words + classes, stencil, lattice values, output
+            return nv * 8 + nv // 16 + nd * (4 + s) + ni * s + n_h * s
+        if name == "Wt_v_bnd":   # voxel words, packed entries, v at anchors, output read-modify-write
+            return nt * (8 + 2 * s) + eb * (2 + s) + nb * s
     if name == "W_mu":
         return nnz * (4 + s) + (n_s + 1) * 4 + 2 * n_s * s + n_h * s
     if name == "Wt_v":
@@ -172,6 +178,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--main-only", action="store_true",
+                    help="run only the --precision complex-state chain (for profiler runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); chains and requests shard across GPUs "
@@ -180,7 +188,7 @@ def main():
     p = build_problem()
     results = {}
     main_chain = None
-    for prec in ("fp32", "fp64"):
+    for prec in ((a.precision,) if a.main_only else ("fp32", "fp64")):
         c = make_chain(p, prec, "compat")
         r = run_chain(c, p, a.steps, a.warmup)
         log(f"[bench] {prec} compat: {r['grad_evals_s']:.1f} grad evals/s, {r['ms_per_iter']:.2f} ms/iter, "
@@ -191,10 +199,12 @@ def main():
             ks, nk = profile_kernels(c, p)
         else:
             c.close()
-    cr = make_chain(p, a.precision, "real")
-    rr = run_chain(cr, p, a.steps, a.warmup)
-    cr.close()
-    log(f"[bench] {a.precision} real: {rr['grad_evals_s']:.1f} grad evals/s")
+    rr = None
+    if not a.main_only:
+        cr = make_chain(p, a.precision, "real")
+        rr = run_chain(cr, p, a.steps, a.warmup)
+        cr.close()
+        log(f"[bench] {a.precision} real: {rr['grad_evals_s']:.1f} grad evals/s")
     # roofline of the dominant kernel
     sb = 4 if a.precision == "fp32" else 8
     dom = max(ks, key=ks.get)
@@ -232,8 +242,8 @@ def main():
             "kriging_unique_solves": int(p["kriging_unique"]), "kriging_build_s": round(p["kriging_s"], 2),
             "W_layout": wst,
         },
-        "fp64_value": round(results["fp64"]["grad_evals_s"], 2),
-        "fp32_real_state_value": round(rr["grad_evals_s"], 2),
+        "fp64_value": round(results["fp64"]["grad_evals_s"], 2) if "fp64" in results else None,
+        "fp32_real_state_value": round(rr["grad_evals_s"], 2) if rr else None,
         "iteration_latency_ms": {"p50": round(r["p50_ms"], 3), "p99": round(r["p99_ms"], 3)},
         "kernel_ms_per_leapfrog_step": {k: round(v, 5) for k, v in ks.items()},
         "fft_passes_hbm_gbs": round(fft_bytes / (fft_ms * 1e-3) / 1e9, 1),

@@ -12,10 +12,12 @@
 //   W (CSR, columns = image-region grid index), WT (transposed CSR)
 // Reference: include/dualres/MultiResParameters.h (whole file).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -123,7 +125,7 @@ struct ChainBase {
   bool data_set = false;
   bool use_cls = false;    // class-stencil W kernels active
   bool force_csr = false;  // keep the plain CSR kernels (tests)
-  int64_t wstats[5] = {0, 0, 0, 0, 0};
+  int64_t wstats[7] = {0, 0, 0, 0, 0, 0, 0};
   // cached energies of the current state A
   bool stats_valid = false;
   double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
@@ -173,9 +175,16 @@ struct Chain : public ChainBase {
   DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
   DevBuf<T> Wvals, WTvals, Ysd, vsd, outbuf;
   DevBuf<double> part, sums;
-  // class-stencil W (see build_classes)
-  DevBuf<int> crg0, crcls, canchor, ccoff, ccn, ccdoff, cvcls, ctcoff, ctcn, ctcdoff;
-  DevBuf<T> ccw, ctcw, Vg;
+  // stencil + boundary W (see build_stencil)
+  struct WStencil {
+    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_dl, vg, vub, wcls;
+    DevBuf<T> w, sub_w, Vl;
+    DevBuf<int> brow, brp, bbase, banchor, tg, trp;
+    DevBuf<uint16_t> bd, td;
+    DevBuf<T> bw, tw, Vb;
+    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0;
+    long nvpad = 0;
+  } ws;
   long pstride = 0;
   int slot_nblk[NSLOT] = {0};
   std::vector<int32_t> out_g;  // region grid index of each output voxel
@@ -520,58 +529,342 @@ struct Chain : public ChainBase {
     if (!h.empty()) DR_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, stream));
   }
 
-  // cols: image-region grid index of every entry; tr*: transposed CSR.
-  bool build_classes(const std::vector<int32_t>& rp, const std::vector<int32_t>& cols, const std::vector<T>& vals,
-                     const std::vector<int32_t>& trp, const std::vector<int32_t>& trows,
-                     const std::vector<T>& tvals, const std::vector<int32_t>& g) {
+  // Stencil + boundary layout of W (kernels k_wmu_int / k_wmu_bnd / k_wtv_int /
+  // k_wtv_bnd).  cols: image-region grid index of every entry.  Returns false
+  // (plain CSR kernels) when a boundary row spans more than 16 voxels on an
+  // axis or has no free anchor position.
+  static long fdiv(long a, long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+  static long fmod_(long a, long b) { return a - b * fdiv(a, b); }
+  bool build_stencil(const std::vector<int32_t>& rp, const std::vector<int32_t>& cols, const std::vector<T>& vals) {
     const long nnz = rp[n_s];
-    // anchors: a distinct grid position per row, taken from its own entries
-    std::vector<uint8_t> claimed(nreg, 0);
-    std::vector<int32_t> anchor(n_s, 0), rg0(n_s, 0);
+    const long R0 = reg[0], R01 = (long)reg[0] * reg[1];
+    auto xyz = [&](long r, long* v) {
+      v[0] = r % R0;
+      v[1] = (r / R0) % reg[1];
+      v[2] = r / R01;
+    };
+    // 1. translation classes of the rows; the largest is the interior stencil
+    std::vector<int32_t> rg0(n_s, 0), doff(nnz);
     for (long i = 0; i < n_s; ++i) {
       if (rp[i + 1] == rp[i]) continue;
       rg0[i] = cols[rp[i]];
-      int32_t an = -1;
-      for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
-        if (!claimed[cols[t]]) {
-          an = cols[t];
-          break;
-        }
-      if (an < 0) return false;  // no free anchor: keep the CSR kernels
-      claimed[an] = 1;
-      anchor[i] = an;
-    }
-    std::vector<int32_t> doff(nnz);
-    for (long i = 0; i < n_s; ++i)
       for (int32_t t = rp[i]; t < rp[i + 1]; ++t) doff[t] = cols[t] - rg0[i];
+    }
     std::vector<int32_t> rcls, coff, cn, cdoff;
     std::vector<T> cw;
     classify(n_s, rp, doff, vals, rcls, coff, cn, cdoff, cw);
+    std::vector<long> cnt(cn.size(), 0);
+    for (long i = 0; i < n_s; ++i)
+      if (rcls[i] >= 0) cnt[rcls[i]]++;
+    int cs = -1;
+    for (int c = 0; c < (int)cn.size(); ++c)
+      if (cs < 0 || cnt[c] > cnt[cs]) cs = c;
+    if (cs >= 0 && cnt[cs] < 2) cs = -1;
+    std::vector<char> interior(n_s, 0);
+    std::vector<int32_t> D;
+    std::vector<T> wD;
+    std::vector<long> d3;  // nD x 3 displacements from the base
+    long dmin[3] = {0, 0, 0}, dmax[3] = {0, 0, 0};
+    if (cs >= 0) {
+      long r = 0;
+      while (rcls[r] != cs) ++r;
+      const int nD = cn[cs];
+      D.assign(cdoff.begin() + coff[cs], cdoff.begin() + coff[cs] + nD);
+      wD.assign(cw.begin() + coff[cs], cw.begin() + coff[cs] + nD);
+      d3.resize(3 * (size_t)nD);
+      long b[3], e[3];
+      xyz(rg0[r], b);
+      for (int s = 0; s < nD; ++s) {
+        xyz((long)rg0[r] + D[s], e);
+        for (int a = 0; a < 3; ++a) {
+          d3[3 * s + a] = e[a] - b[a];
+          dmin[a] = std::min(dmin[a], e[a] - b[a]);
+          dmax[a] = std::max(dmax[a], e[a] - b[a]);
+        }
+      }
+      // a row is interior when its stencil stays inside the region grid on
+      // every axis (then flat offsets and 3D displacements agree)
+      for (long i = 0; i < n_s; ++i) {
+        if (rcls[i] != cs) continue;
+        xyz(rg0[i], b);
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) ok = ok && b[a] + dmin[a] >= 0 && b[a] + dmax[a] < reg[a];
+        interior[i] = ok;
+      }
+    }
+    std::vector<int32_t> irow, ibase;
+    for (long i = 0; i < n_s; ++i)
+      if (interior[i]) {
+        irow.push_back((int32_t)i);
+        ibase.push_back(rg0[i]);
+      }
+    const long n_int = (long)irow.size();
+    // 2. lattice of the interior bases and the residue-class sub-stencils
+    long c[3] = {0, 0, 0}, sg[3] = {1, 1, 1};
+    if (n_int > 0) {
+      xyz(ibase[0], c);
+      long gcd_[3] = {0, 0, 0}, b[3];
+      for (long k = 0; k < n_int; ++k) {
+        xyz(ibase[k], b);
+        for (int a = 0; a < 3; ++a) gcd_[a] = std::gcd(gcd_[a], std::labs(b[a] - c[a]));
+      }
+      for (int a = 0; a < 3; ++a) {
+        sg[a] = gcd_[a] > 0 ? std::min<long>(gcd_[a], 64) : 1;
+        c[a] = fmod_(c[a], sg[a]);
+      }
+      for (long k = 0; k < n_int; ++k) {  // gcd capped: re-check membership
+        xyz(ibase[k], b);
+        for (int a = 0; a < 3; ++a)
+          if (fmod_(b[a] - c[a], sg[a]) != 0) return false;
+      }
+    }
+    const int nD = (int)D.size();
+    long lo[3], hi[3], L[3];
+    for (int a = 0; a < 3; ++a) {
+      const long ubmin = fdiv(-c[a], sg[a]), ubmax = fdiv(reg[a] - 1 - c[a], sg[a]);
+      long omin = 0, omax = 0;
+      for (int s = 0; s < nD; ++s) {
+        const long o = -fdiv(d3[3 * s + a], sg[a]);
+        omin = std::min(omin, o);
+        omax = std::max(omax, o);
+      }
+      lo[a] = ubmin + omin;
+      hi[a] = ubmax + omax;
+      L[a] = hi[a] - lo[a] + 1;
+    }
+    if ((long)D.size() * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of k_wmu_int
+    const long nVl = L[0] * L[1] * L[2];
+    DR_REQUIRE(nVl < (1L << 31), DR_EDOMAIN, "W stencil lattice too large");
+    auto flatL = [&](const long* u) { return (u[0] - lo[0]) + L[0] * ((u[1] - lo[1]) + L[1] * (u[2] - lo[2])); };
+    const int ncls = (int)(sg[0] * sg[1] * sg[2]);
+    std::vector<int32_t> sub_start(ncls + 1, 0), sub_dl(nD);
+    std::vector<T> sub_w(nD);
+    {
+      std::vector<int> scls(nD);
+      for (int s = 0; s < nD; ++s) {
+        long rho[3];
+        for (int a = 0; a < 3; ++a) rho[a] = fmod_(d3[3 * s + a], sg[a]);
+        scls[s] = (int)(rho[0] + sg[0] * (rho[1] + sg[1] * rho[2]));
+        sub_start[scls[s] + 1]++;
+      }
+      for (int k = 0; k < ncls; ++k) sub_start[k + 1] += sub_start[k];
+      std::vector<int32_t> fill(sub_start.begin(), sub_start.end() - 1);
+      for (int s = 0; s < nD; ++s) {  // stencil order within a class
+        long o[3];
+        for (int a = 0; a < 3; ++a) o[a] = -fdiv(d3[3 * s + a], sg[a]);
+        const int k = fill[scls[s]]++;
+        sub_dl[k] = (int32_t)(o[0] + L[0] * (o[1] + L[1] * o[2]));
+        sub_w[k] = wD[s];
+      }
+    }
+    std::vector<int32_t> ilidx(n_int);
+    for (long k = 0; k < n_int; ++k) {
+      long b[3], u[3];
+      xyz(ibase[k], b);
+      for (int a = 0; a < 3; ++a) u[a] = (b[a] - c[a]) / sg[a];
+      ilidx[k] = (int32_t)flatL(u);
+    }
+    // 3. data voxels bucketed by residue class, buckets padded to wavefronts
+    std::vector<std::vector<int32_t>> bg(ncls), bu(ncls);
+    long safe_ub = -1;
+    std::vector<int32_t> gidx(n_h);
+    for (long d = 0; d < n_h; ++d) {
+      gidx[d] = (int32_t)region_index(ijk[3 * d], ijk[3 * d + 1], ijk[3 * d + 2]);
+      long x[3], u[3], rho[3];
+      xyz(gidx[d], x);
+      for (int a = 0; a < 3; ++a) {
+        rho[a] = fmod_(x[a] - c[a], sg[a]);
+        u[a] = fdiv(x[a] - c[a], sg[a]);
+      }
+      const int k = (int)(rho[0] + sg[0] * (rho[1] + sg[1] * rho[2]));
+      const long f = flatL(u);
+      if (safe_ub < 0) safe_ub = f;
+      bg[k].push_back(gidx[d]);
+      bu[k].push_back((int32_t)f);
+    }
+    if (safe_ub < 0) safe_ub = 0;
+    std::vector<int32_t> vg, vub, wcls;
+    for (int k = 0; k < ncls; ++k) {
+      if (bg[k].empty()) continue;
+      vg.insert(vg.end(), bg[k].begin(), bg[k].end());
+      vub.insert(vub.end(), bu[k].begin(), bu[k].end());
+      while (vg.size() % 64) {
+        vg.push_back(-1);
+        vub.push_back((int32_t)safe_ub);
+      }
+      for (size_t w = 0; w < bg[k].size(); w += 64) wcls.push_back(k);
+    }
+    while (vg.size() % kThreads || vg.empty()) {
+      vg.push_back(-1);
+      vub.push_back((int32_t)safe_ub);
+      if (vg.size() % 64 == 1) wcls.push_back(0);
+    }
+    if (wcls.size() * 64 < vg.size()) wcls.resize(vg.size() / 64, 0);
+    // 4. boundary rows: packed offsets from the row's bounding-box corner and
+    //    a distinct anchor position (one of the row's own entries) for v
+    std::vector<int32_t> brow, brp{0}, bbase, banchor;
+    std::vector<uint16_t> bd;
+    std::vector<T> bw;
+    std::vector<uint8_t> claimed(nreg, 0);
+    std::vector<int32_t> row_anchor(n_s, -1);
+    long nb_real = 0;
+    for (long i = 0; i < n_s; ++i) {
+      if (interior[i]) continue;
+      long mn[3] = {LONG_MAX, LONG_MAX, LONG_MAX}, mx[3] = {LONG_MIN, LONG_MIN, LONG_MIN}, x[3];
+      for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
+        xyz(cols[t], x);
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = std::min(mn[a], x[a]);
+          mx[a] = std::max(mx[a], x[a]);
+        }
+      }
+      int32_t an = -1;
+      if (rp[i + 1] > rp[i]) {
+        for (int a = 0; a < 3; ++a)
+          if (mx[a] - mn[a] > 15) return false;
+        for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
+          if (!claimed[cols[t]]) {
+            an = cols[t];
+            break;
+          }
+        if (an < 0) return false;
+        claimed[an] = 1;
+      }
+      row_anchor[i] = an;
+      const long base = rp[i + 1] > rp[i] ? mn[0] + R0 * mn[1] + R01 * mn[2] : 0;
+      nb_real += rp[i + 1] - rp[i];
+      for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
+        xyz(cols[t], x);
+        bd.push_back((uint16_t)((x[0] - mn[0]) | ((x[1] - mn[1]) << 5) | ((x[2] - mn[2]) << 10)));
+        bw.push_back(vals[t]);
+      }
+      while (bw.size() % 4) {  // whole quads: offset 0 (the base), weight 0
+        bd.push_back(0);
+        bw.push_back((T)0);
+      }
+      brow.push_back((int32_t)i);
+      bbase.push_back((int32_t)base);
+      banchor.push_back(an);
+      brp.push_back((int32_t)bw.size());
+    }
+    // 5. transposed boundary entries per touched data voxel, in row order
+    std::vector<int32_t> tcount(n_h + 1, 0);
+    for (long i = 0; i < n_s; ++i)
+      if (!interior[i])
+        for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) tcount[W_cols[t] + 1]++;
+    std::vector<int32_t> tstart(n_h + 1, 0);
+    for (long d = 0; d < n_h; ++d) tstart[d + 1] = tstart[d] + ((tcount[d + 1] + 3) & ~3);
+    const long nbt = tstart[n_h];
+    const uint16_t qself = (uint16_t)(15 | (15 << 5) | (15 << 10));  // offset 0: the voxel itself
+    std::vector<uint16_t> tdall(nbt, qself);
+    std::vector<T> twall(nbt, (T)0);
+    {
+      std::vector<int32_t> fill(tstart.begin(), tstart.end() - 1);
+      for (long i = 0; i < n_s; ++i) {
+        if (interior[i]) continue;
+        long xa[3], x[3];
+        xyz(row_anchor[i] >= 0 ? row_anchor[i] : 0, xa);
+        for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) {
+          const int32_t dv = W_cols[t];
+          xyz(gidx[dv], x);
+          long q[3];
+          for (int a = 0; a < 3; ++a) q[a] = x[a] - xa[a] + 15;
+          for (int a = 0; a < 3; ++a)
+            if (q[a] < 0 || q[a] > 31) return false;
+          tdall[fill[dv]] = (uint16_t)(q[0] | (q[1] << 5) | (q[2] << 10));
+          twall[fill[dv]] = vals[t];
+          fill[dv]++;
+        }
+      }
+    }
+    std::vector<int32_t> tg, trp{0};
     for (long d = 0; d < n_h; ++d)
-      for (int32_t t = trp[d]; t < trp[d + 1]; ++t) doff[t] = g[d] - anchor[trows[t]];
-    std::vector<int32_t> vcls, tcoff, tcn, tcdoff;
-    std::vector<T> tcw;
-    classify(n_h, trp, doff, tvals, vcls, tcoff, tcn, tcdoff, tcw);
-    upload(crg0, rg0);
-    upload(crcls, rcls);
-    upload(canchor, anchor);
-    upload(ccoff, coff);
-    upload(ccn, cn);
-    upload(ccdoff, cdoff);
-    upload(ccw, cw);
-    upload(cvcls, vcls);
-    upload(ctcoff, tcoff);
-    upload(ctcn, tcn);
-    upload(ctcdoff, tcdoff);
-    upload(ctcw, tcw);
-    Vg.alloc(nreg);
-    Vg.zero(stream);
+      if (tstart[d + 1] > tstart[d]) {
+        tg.push_back(gidx[d]);
+        trp.push_back(tstart[d + 1]);
+      }
+    // 6. every device access of the four kernels stays inside its buffer
+    {
+      long fmin = 0, fmax = 0;
+      for (int s = 0; s < nD; ++s) {
+        fmin = std::min(fmin, (long)D[s]);
+        fmax = std::max(fmax, (long)D[s]);
+      }
+      for (long k = 0; k < n_int; ++k) {
+        DR_REQUIRE(ilidx[k] >= 0 && ilidx[k] < nVl, DR_ELOGIC, "W stencil: lattice index out of range");
+        DR_REQUIRE((long)ibase[k] + fmin >= 0 && (long)ibase[k] + fmax < nreg, DR_ELOGIC,
+                   "W stencil: interior entry outside the region");
+      }
+    }
+    {
+      std::vector<long> smin(ncls, LONG_MAX), smax(ncls, LONG_MIN);
+      for (int k = 0; k < ncls; ++k)
+        for (int t = sub_start[k]; t < sub_start[k + 1]; ++t) {
+          smin[k] = std::min(smin[k], (long)sub_dl[t]);
+          smax[k] = std::max(smax[k], (long)sub_dl[t]);
+        }
+      for (size_t t = 0; t < vg.size(); ++t) {
+        const int k = wcls[t / 64];
+        DR_REQUIRE(k >= 0 && k < ncls, DR_ELOGIC, "W stencil: residue class out of range");
+        if (sub_start[k + 1] > sub_start[k])
+          DR_REQUIRE(vub[t] + smin[k] >= 0 && vub[t] + smax[k] < nVl, DR_ELOGIC,
+                     "W stencil: sub-stencil read outside the lattice grid");
+        DR_REQUIRE(vg[t] < nreg, DR_ELOGIC, "W stencil: voxel outside the region");
+      }
+    }
+    for (size_t k = 0; k < brow.size(); ++k) {
+      DR_REQUIRE(banchor[k] < nreg, DR_ELOGIC, "W stencil: anchor outside the region");
+      for (int32_t t = brp[k]; t < brp[k + 1]; ++t) {
+        const long q = bd[t], j = (long)bbase[k] + (q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10);
+        DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: boundary entry outside the region");
+      }
+    }
+    for (size_t k = 0; k < tg.size(); ++k)
+      for (int32_t t = trp[k]; t < trp[k + 1]; ++t) {
+        const long q = tdall[t], j = (long)tg[k] + (15 + 15 * R0 + 15 * R01) -
+                                     ((q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10));
+        DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: transposed entry outside the region");
+      }
+    // 7. upload
+    ws.n_int = (int)n_int;
+    ws.nD = nD;
+    ws.n_bnd = (int)brow.size();
+    ws.n_touch = (int)tg.size();
+    ws.dc = (int)(15 + 15 * R0 + 15 * R01);
+    ws.nvpad = (long)vg.size();
+    upload(ws.irow, irow);
+    upload(ws.ibase, ibase);
+    upload(ws.ilidx, ilidx);
+    upload(ws.doff, D);
+    upload(ws.w, wD);
+    upload(ws.sub_start, sub_start);
+    upload(ws.sub_dl, sub_dl);
+    upload(ws.sub_w, sub_w);
+    upload(ws.vg, vg);
+    upload(ws.vub, vub);
+    upload(ws.wcls, wcls);
+    upload(ws.brow, brow);
+    upload(ws.brp, brp);
+    upload(ws.bbase, bbase);
+    upload(ws.banchor, banchor);
+    upload(ws.bd, bd);
+    upload(ws.bw, bw);
+    upload(ws.tg, tg);
+    upload(ws.trp, trp);
+    upload(ws.td, tdall);
+    upload(ws.tw, twall);
+    ws.Vl.alloc(std::max<long>(nVl, 1));
+    ws.Vl.zero(stream);
+    ws.Vb.alloc(std::max<long>(nreg, 1));
+    ws.Vb.zero(stream);
     DR_HIP(hipStreamSynchronize(stream));
-    wstats[0] = (int64_t)cn.size();
-    wstats[1] = (int64_t)cdoff.size();
-    wstats[2] = (int64_t)tcn.size();
-    wstats[3] = (int64_t)tcdoff.size();
+    wstats[0] = n_int;
+    wstats[1] = nD;
+    wstats[2] = (int64_t)brow.size();
+    wstats[3] = nb_real;
     wstats[4] = nnz;
+    wstats[5] = (int64_t)tg.size();
+    wstats[6] = (int64_t)vg.size();
     return true;
   }
 
@@ -588,7 +881,7 @@ struct Chain : public ChainBase {
     DR_HIP(hipMemcpyAsync(Yg.p, yg.data(), nreg * sizeof(T), hipMemcpyHostToDevice, stream));
     DR_HIP(hipMemcpyAsync(mask.p, mk_.data(), nreg, hipMemcpyHostToDevice, stream));
     if (cfg.n_datasets == 2) {
-      const long need = cdiv(n_s, kThreads / 64) + 1;
+      const long need = cdiv(n_s, kThreads / 64) + cdiv(n_s * 8, kThreads) + 2;
       if (need > pstride) {
         pstride = need;
         part.alloc((size_t)NSLOT * pstride);
@@ -598,25 +891,13 @@ struct Chain : public ChainBase {
       DR_REQUIRE(nnz < (1L << 31), DR_EDOMAIN, "W has too many nonzeros for 32-bit indices");
       std::vector<int32_t> rp(n_s + 1), cols(nnz);
       std::vector<T> vals(nnz);
-      std::vector<int32_t> trp(n_h + 1, 0), trows(nnz);
-      std::vector<T> tvals(nnz);
       for (long i = 0; i <= n_s; ++i) rp[i] = (int32_t)W_rp[i];
       for (long j = 0; j < nnz; ++j) {
         const int32_t c = W_cols[j];
         DR_REQUIRE(c >= 0 && c < n_h, DR_EDOMAIN, "W column index out of range");
         cols[j] = g[c];
         vals[j] = (T)W_vals[j];
-        trp[c + 1]++;
       }
-      for (long d = 0; d < n_h; ++d) trp[d + 1] += trp[d];
-      std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
-      for (long i = 0; i < n_s; ++i)
-        for (long j = W_rp[i]; j < W_rp[i + 1]; ++j) {
-          const int32_t c = W_cols[j];
-          trows[fill[c]] = (int32_t)i;
-          tvals[fill[c]] = (T)W_vals[j];
-          fill[c]++;
-        }
       Ysd.alloc(n_s);
       vsd.alloc(n_s);
       WTv.alloc(nreg);
@@ -624,8 +905,21 @@ struct Chain : public ChainBase {
       std::vector<T> ys(n_s);
       for (long i = 0; i < n_s; ++i) ys[i] = (T)Ys[i];
       DR_HIP(hipMemcpyAsync(Ysd.p, ys.data(), n_s * sizeof(T), hipMemcpyHostToDevice, stream));
-      use_cls = !force_csr && build_classes(rp, cols, vals, trp, trows, tvals, g);
+      use_cls = !force_csr && build_stencil(rp, cols, vals);
       if (!use_cls) {
+        // transposed CSR for W^T v, rows ascending within each column
+        std::vector<int32_t> trp(n_h + 1, 0), trows(nnz);
+        std::vector<T> tvals(nnz);
+        for (long j = 0; j < nnz; ++j) trp[W_cols[j] + 1]++;
+        for (long d = 0; d < n_h; ++d) trp[d + 1] += trp[d];
+        std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+        for (long i = 0; i < n_s; ++i)
+          for (long j = W_rp[i]; j < W_rp[i + 1]; ++j) {
+            const int32_t c = W_cols[j];
+            trows[fill[c]] = (int32_t)i;
+            tvals[fill[c]] = (T)W_vals[j];
+            fill[c]++;
+          }
         Wrp.alloc(n_s + 1);
         Wcols.alloc(std::max<long>(nnz, 1));
         Wvals.alloc(std::max<long>(nnz, 1));
@@ -675,20 +969,38 @@ struct Chain : public ChainBase {
   void w_products(const T* R, int slot) {
     if (cfg.n_datasets != 2) return;
     const int g1 = cdiv(n_s, kThreads / 64);
-    if (slot >= 0) note_blocks(slot, g1);
     const int g2 = cdiv(n_h, kThreads / 16);
     if (use_cls) {
-      hipLaunchKernelGGL(k_wmu_cls<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, crg0.p, crcls.p,
-                         WClasses{ccoff.p, ccn.p, ccdoff.p}, ccw.p, R, Ysd.p, (T)sigma[1], canchor.p, Vg.p,
-                         slot >= 0 ? pslot(slot) : nullptr);
+      const int gi = cdiv((long)ws.n_int * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64);
+      if (slot >= 0) note_blocks(slot, gi + gb);
+      double* pp = slot >= 0 ? pslot(slot) : nullptr;
+      const int R0 = reg[0], R01 = reg[0] * reg[1];
+      if (ws.n_int > 0) {
+        hipLaunchKernelGGL(k_wmu_int<T>, dim3(gi), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream, ws.n_int, ws.irow.p, ws.ibase.p,
+                           ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, R, Ysd.p, (T)sigma[1], ws.Vl.p, pp);
+        DR_HIP(hipGetLastError());
+      }
+      mark();
+      if (ws.n_bnd > 0) {
+        hipLaunchKernelGGL(k_wmu_bnd<T>, dim3(gb), dim3(kThreads), 0, stream, ws.n_bnd, ws.brow.p, ws.brp.p,
+                           ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01, R, Ysd.p, (T)sigma[1], ws.Vb.p,
+                           pp ? pp + gi : nullptr);
+        DR_HIP(hipGetLastError());
+      }
+      mark();
+      hipLaunchKernelGGL(k_wtv_int<T>, dim3((unsigned)(ws.nvpad / kThreads)), dim3(kThreads), 0, stream, ws.vg.p,
+                         ws.vub.p, ws.wcls.p, ws.sub_start.p, ws.sub_dl.p, ws.sub_w.p, ws.Vl.p, WTv.p);
       DR_HIP(hipGetLastError());
       mark();
-      hipLaunchKernelGGL(k_wtv_cls<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, gI.p, cvcls.p,
-                         WClasses{ctcoff.p, ctcn.p, ctcdoff.p}, ctcw.p, Vg.p, WTv.p);
-      DR_HIP(hipGetLastError());
+      if (ws.n_touch > 0) {
+        hipLaunchKernelGGL(k_wtv_bnd<T>, dim3(cdiv(ws.n_touch, kThreads / 16)), dim3(kThreads), 0, stream,
+                           ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTv.p);
+        DR_HIP(hipGetLastError());
+      }
       mark();
       return;
     }
+    if (slot >= 0) note_blocks(slot, g1);
     hipLaunchKernelGGL(k_wmu<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, Wrp.p, Wcols.p, Wvals.p, R, Ysd.p,
                        (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr);
     DR_HIP(hipGetLastError());
@@ -1109,7 +1421,7 @@ int dr_w_stats(dr_chain* c, int64_t* stats) {
   return guarded([&] {
     auto& k = *c->impl;
     stats[0] = k.use_cls ? 1 : 0;
-    for (int i = 0; i < 5; ++i) stats[1 + i] = k.wstats[i];
+    for (int i = 0; i < 7; ++i) stats[1 + i] = k.wstats[i];
   });
 }
 

@@ -354,63 +354,152 @@ k_wtv(int n_h, const int* __restrict__ rp, const int* __restrict__ rows, const T
   if (d < n_h && lane == 0) out[gidx[d]] = acc;
 }
 
-// Class-stencil form of the same two products.  Rows of W that are equal up
-// to a translation on the image-region grid (same relative offsets, same
-// weights) share one class; a row stores only its anchor grid index and class.
-// Interior kriging rows all fall in one class, so their weights and offsets
-// stay in cache and only the boundary rows stream explicit entries.  Entry
-// order within a row is the CSR order, so sums match k_wmu / k_wtv bitwise.
-struct WClasses {
-  const int* off;   // first entry of each class
-  const int* n;     // entries per class
-  const int* doff;  // grid offset of each entry relative to the anchor
-};
+// Stencil + boundary form of the same two products.  The rows of W that are
+// equal up to a translation of the image-region grid and make up the largest
+// such class ("interior" rows: every neighbour present) share one stencil of
+// nD (region-grid offset, weight) pairs.  Their bases (first entry) lie on a
+// lattice c + sigma*Z^3, so W^T v restricted to them is, at a voxel x, a sum
+// over only the stencil entries whose 3D displacement is congruent to x - c
+// modulo sigma (the voxel's residue class), read from a grid Vl holding
+// v at lattice coordinates.  All other rows ("boundary" rows) keep explicit
+// entries with 3D offsets packed in 15 bits (5 per axis).
 
+// packed 3D offset (dx | dy << 5 | dz << 10) -> region-grid flat offset
+__device__ __forceinline__ int unpack3(unsigned q, int R0, int R01) {
+  return (int)(q & 31u) + R0 * (int)((q >> 5) & 31u) + R01 * (int)(q >> 10);
+}
+
+// four packed entries: offsets (4 x 16 bits) and weights, read as one vector each
+template <typename T>
+struct alignas(4 * sizeof(T)) Quad {
+  T v[4];
+};
+__device__ __forceinline__ unsigned lo16(unsigned x) { return x & 0xffffu; }
+__device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
+
+// interior rows: 8 lanes per row, lane q takes stencil entries q, q+8, ...
+// (consecutive lanes read consecutive entries of an x-run), then a fixed
+// butterfly over the 8 lanes
+// (stencil table staged in LDS: nD * (4 + sizeof(T)) bytes of dynamic LDS)
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_wmu_cls(int n_s, const int* __restrict__ rg0, const int* __restrict__ rcls, WClasses C, const T* __restrict__ w,
-          const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, const int* __restrict__ anchor,
-          T* __restrict__ V, double* part) {
+k_wmu_int(int n, const int* __restrict__ row, const int* __restrict__ base, const int* __restrict__ lidx, int nD,
+          const int* __restrict__ doff, const T* __restrict__ w, const T* __restrict__ R, const T* __restrict__ Ys,
+          T sig_s, T* __restrict__ Vl, double* part) {
+  __shared__ double red[kThreads];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* sw = reinterpret_cast<T*>(lds_raw);
+  int* so = reinterpret_cast<int*>(lds_raw + (size_t)nD * sizeof(T));
+  for (int s = threadIdx.x; s < nD; s += kThreads) {
+    sw[s] = w[s];
+    so[s] = doff[s];
+  }
+  __syncthreads();
+  const int q = threadIdx.x & 7;
+  const int k = (blockIdx.x * kThreads + threadIdx.x) >> 3;
+  T acc = (T)0;
+  if (k < n) {
+    const T* Rb = R + base[k];
+#pragma unroll 4
+    for (int s = q; s < nD; s += 8) acc += sw[s] * Rb[so[s]];
+  }
+  acc += __shfl_xor(acc, 1, 8);
+  acc += __shfl_xor(acc, 2, 8);
+  acc += __shfl_xor(acc, 4, 8);
+  double e2 = 0.0;
+  if (k < n && q == 0) {
+    const T e = Ys[row[k]] - acc;
+    Vl[lidx[k]] = sig_s * e;
+    e2 = (double)e * (double)e;
+  }
+  if (part) red_write(e2, red, part, blockIdx.x);
+}
+
+// boundary rows, one wavefront per row; entry lists start on multiples of 4
+// and are padded to whole quads (offset 0, weight 0)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wmu_bnd(int n, const int* __restrict__ row, const int* __restrict__ rp, const int* __restrict__ base,
+          const uint16_t* __restrict__ d, const T* __restrict__ w, const int* __restrict__ anchor, int R0, int R01,
+          const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, T* __restrict__ Vb, double* part) {
   __shared__ double red[kThreads];
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int k = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   double e2 = 0.0;
-  if (row < n_s) {
-    const int c = rcls[row];
+  if (k < n) {
+    const int j1 = rp[k + 1] >> 2;
+    const T* Rb = R + base[k];
+    const uint2* d4 = reinterpret_cast<const uint2*>(d);
+    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(w);
     T acc = (T)0;
-    if (c >= 0) {
-      const int base = rg0[row], o = C.off[c], n = C.n[c];
-      for (int j = lane; j < n; j += 64) acc += w[o + j] * R[base + C.doff[o + j]];
+    for (int j = (rp[k] >> 2) + lane; j < j1; j += 64) {
+      const uint2 dq = d4[j];
+      const Quad<T> wq = w4[j];
+      acc += wq.v[0] * Rb[unpack3(lo16(dq.x), R0, R01)];
+      acc += wq.v[1] * Rb[unpack3(hi16(dq.x), R0, R01)];
+      acc += wq.v[2] * Rb[unpack3(lo16(dq.y), R0, R01)];
+      acc += wq.v[3] * Rb[unpack3(hi16(dq.y), R0, R01)];
     }
     acc = wave_sum(acc);
     if (lane == 0) {
-      const T e = Ys[row] - acc;
-      if (c >= 0) V[anchor[row]] = sig_s * e;
+      const T e = Ys[row[k]] - acc;
+      const int a = anchor[k];
+      if (a >= 0) Vb[a] = sig_s * e;
       e2 = (double)e * (double)e;
     }
   }
   if (part) red_write(e2, red, part, blockIdx.x);
 }
 
+// W^T v of the interior rows at every data voxel (written, not added).  The
+// voxel list is bucketed by residue class and each bucket padded to whole
+// wavefronts, so a wavefront reads one sub-stencil with scalar loads; padding
+// lanes (g < 0) compute on a valid lattice base and store nothing.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_wtv_cls(int n_h, const int* __restrict__ vg, const int* __restrict__ vcls, WClasses C, const T* __restrict__ w,
-          const T* __restrict__ V, T* __restrict__ out) {
-  const int lane = threadIdx.x & 15;
-  const int d = blockIdx.x * (kThreads / 16) + (threadIdx.x >> 4);
+k_wtv_int(const int* __restrict__ g, const int* __restrict__ ub, const int* __restrict__ wcls,
+          const int* __restrict__ sub_start, const int* __restrict__ sub_dl, const T* __restrict__ sub_w,
+          const T* __restrict__ Vl, T* __restrict__ out) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  const int cls = __builtin_amdgcn_readfirstlane(wcls[t >> 6]);
+  const int s0 = sub_start[cls], s1 = sub_start[cls + 1];
+  const T* V = Vl + ub[t];
   T acc = (T)0;
-  int g = 0;
-  if (d < n_h) {
-    const int c = vcls[d];
-    g = vg[d];
-    if (c >= 0) {
-      const int o = C.off[c], n = C.n[c];
-      for (int j = lane; j < n; j += 16) acc += w[o + j] * V[g - C.doff[o + j]];
+#pragma unroll 4
+  for (int s = s0; s < s1; ++s) acc += sub_w[s] * V[sub_dl[s]];
+  const int gj = g[t];
+  if (gj >= 0) out[gj] = acc;
+}
+
+// W^T v of the boundary rows, added at the voxels they touch; 16 lanes per
+// voxel, quads of entries as in k_wmu_bnd (pad offset = the voxel itself,
+// weight 0).  Entry offsets are voxel - anchor + 15 per axis, packed.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wtv_bnd(int n, const int* __restrict__ g, const int* __restrict__ rp, const uint16_t* __restrict__ d,
+          const T* __restrict__ w, int R0, int R01, int dc, const T* __restrict__ Vb, T* __restrict__ out) {
+  const int lane = threadIdx.x & 15;
+  const int k = blockIdx.x * (kThreads / 16) + (threadIdx.x >> 4);
+  T acc = (T)0;
+  int gj = 0;
+  if (k < n) {
+    gj = g[k];
+    const int j1 = rp[k + 1] >> 2;
+    const T* Vg = Vb + (gj + dc);
+    const uint2* d4 = reinterpret_cast<const uint2*>(d);
+    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(w);
+    for (int j = (rp[k] >> 2) + lane; j < j1; j += 16) {
+      const uint2 dq = d4[j];
+      const Quad<T> wq = w4[j];
+      acc += wq.v[0] * Vg[-unpack3(lo16(dq.x), R0, R01)];
+      acc += wq.v[1] * Vg[-unpack3(hi16(dq.x), R0, R01)];
+      acc += wq.v[2] * Vg[-unpack3(lo16(dq.y), R0, R01)];
+      acc += wq.v[3] * Vg[-unpack3(hi16(dq.y), R0, R01)];
     }
   }
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
-  if (d < n_h && lane == 0) out[g] = acc;
+  if (k < n && lane == 0) out[gj] += acc;
 }
 
 // sum over the half spectrum of w |X|^2 a(k) with a = invlam (prior) or

@@ -96,14 +96,15 @@ class Chain:
             L.check(self._lib.dr_set_data(self._h, L.dptr(Yh), Yh.size, None, 0, None, None, None))
 
     def w_layout(self, force_csr):
-        """Select plain CSR (True) or the class-stencil layout (False, default) before set_data."""
+        """Select plain CSR (True) or the stencil + boundary layout (False, default) before set_data."""
         L.check(self._lib.dr_w_layout(self._h, 1 if force_csr else 0))
 
     def w_stats(self):
-        st = np.zeros(6, dtype=np.int64)
+        st = np.zeros(8, dtype=np.int64)
         L.check(self._lib.dr_w_stats(self._h, L.iptr64(st)))
-        return {"class_layout": bool(st[0]), "row_classes": int(st[1]), "row_class_entries": int(st[2]),
-                "voxel_classes": int(st[3]), "voxel_class_entries": int(st[4]), "nnz": int(st[5])}
+        return {"stencil_layout": bool(st[0]), "interior_rows": int(st[1]), "stencil_entries": int(st[2]),
+                "boundary_rows": int(st[3]), "boundary_entries": int(st[4]), "nnz": int(st[5]),
+                "touched_voxels": int(st[6]), "padded_voxels": int(st[7])}
 
     def add_data_to_mu(self):
         L.check(self._lib.dr_add_data_to_mu(self._h))
@@ -203,7 +204,10 @@ class Chain:
         L.check(self._lib.dr_profile_kernels(self._h, L.dptr(ms), 16, C.byref(nk), C.byref(ns)))
         names = ["z_fft_kick_drift", "c2r_y", "c2r_x"]
         if self.n_datasets == 2:
-            names += ["W_mu", "Wt_v"]
+            if self.w_stats()["stencil_layout"]:
+                names += ["W_mu_int", "W_mu_bnd", "Wt_v_int", "Wt_v_bnd"]
+            else:
+                names += ["W_mu", "Wt_v"]
         names += ["r2c_x_residual", "r2c_y"]
         return dict(zip(names, ms[: nk.value].tolist())), ns.value
 

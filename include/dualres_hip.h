@@ -101,13 +101,17 @@ int dr_get_lambda(dr_chain* c, double* lambda_out);
 int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, int64_t n_s,
                 const int64_t* W_rowptr, const int32_t* W_cols, const double* W_vals);
 
-/* Device layout of W.  By default rows of W (and of W^T) that are equal up
- * to a translation on the image grid share one stored class (same offsets,
- * same weights, same summation order as CSR); force_csr = 1 keeps plain CSR.
- * Call before dr_set_data.  No counterpart in the reference. */
+/* Device layout of W.  By default the largest class of rows that are equal
+ * up to a translation of the image grid (the interior kriging rows) is stored
+ * once as a stencil, and only the other rows keep explicit entries (3D
+ * offsets packed in 16 bits); W^T v of the stencil rows runs per residue class
+ * of the stencil lattice.  force_csr = 1 keeps plain CSR / transposed CSR.
+ * Sums differ from CSR only in association order.  Call before dr_set_data.
+ * No counterpart in the reference. */
 int dr_w_layout(dr_chain* c, int force_csr);
-/* stats[0] = class layout active; [1] row classes; [2] row-class entries;
- * [3] voxel (W^T) classes; [4] voxel-class entries; [5] nnz(W). */
+/* stats[0] = stencil layout active; [1] interior (stencil) rows; [2] stencil
+ * entries; [3] boundary rows; [4] boundary entries; [5] nnz(W); [6] data
+ * voxels touched by boundary rows; [7] padded voxel slots of the W^T pass. */
 int dr_w_stats(dr_chain* c, int64_t* stats);
 
 /* MultiResParameters::add_data_to_mu (:631-641). */
