@@ -30,6 +30,7 @@ THETA = (0.806, 0.132, 1.5)
 RADIUS = 6.9
 EMBED = (256, 320, 256)
 L_STEPS = 25
+THIN = 3       # the reference's saves every third post-burn-in iteration at 1 mm
 
 
 def log(*a):
@@ -65,14 +66,14 @@ def run_chain(c, p, steps, warmup, seed=8675309):
     from dualres_amd.sampler import HMCParameters, McmcSummary, gibbs_sigma, init_sigma
     rng = np.random.default_rng(seed)
     init_sigma(c, rng, THETA[0])
-    hmc = HMCParameters(burnin=warmup, n_save=steps, thin=1, integrator_steps=L_STEPS, starting_eps=1.0 / L_STEPS,
-                        mh_target=0.65)
-    summ = McmcSummary(c.n_out, 2)
+    hmc = HMCParameters(burnin=warmup, n_save=max(1, -(-steps // THIN)), thin=THIN, integrator_steps=L_STEPS,
+                        starting_eps=1.0 / L_STEPS, mh_target=0.65)
+    summ = McmcSummary(c.n_out, 2, chain=c)
     n_s = p["Ys"].size
     lat = []
     grads = 0
     t0 = None
-    while hmc.iteration < hmc.max_iterations():
+    while hmc.iteration < warmup + steps:
         timed = hmc.iteration >= warmup
         if timed and t0 is None:
             t0 = time.perf_counter()
@@ -83,8 +84,7 @@ def run_chain(c, p, steps, warmup, seed=8675309):
         hmc.update(r["mh_rate"])
         s = gibbs_sigma(c, rng, c.n_h, n_s, r["sse_h"], r["sse_s"])
         if hmc.save_iteration():
-            mu = c.mu_()
-            summ.update(mu, np.sqrt(1.0 / np.asarray(s)), -0.5 * (s[0] * r["sse_h"] + s[1] * r["sse_s"]))
+            summ.update(None, np.sqrt(1.0 / np.asarray(s)), -0.5 * (s[0] * r["sse_h"] + s[1] * r["sse_s"]))
         if timed:
             grads += L + 1
             lat.append(time.perf_counter() - ti)
@@ -124,12 +124,13 @@ def kernel_bytes(name, p, prec_bytes, c):
     ws = c.w_stats()
     if ws["stencil_layout"]:
         ni, nd, nb, eb = ws["interior_rows"], ws["stencil_entries"], ws["boundary_rows"], ws["boundary_entries"]
-        nt, nv = ws["touched_voxels"], ws["padded_voxels"]
+        nt = ws["touched_voxels"]
         if name == "W_mu_int":   # row/base/lattice words, Ys, v, the stencil, the region field R
             return ni * (12 + 2 * s) + nd * (4 + s) + n_h * s
         if name == "W_mu_bnd":   # row words, Ys, v, packed entries, R at the touched voxels
             return nb * (16 + 2 * s) + eb * (2 + s) + nt * s
         if name == "Wt_v_int":   # voxel slot words + classes, stencil, lattice values, output
+            nv = ws["padded_voxels"]
             return nv * 8 + nv // 16 + nd * (4 + s) + ni * s + n_h * s
         if name == "Wt_v_bnd":   # voxel words, packed entries, v at anchors, output read-modify-write
             return nt * (8 + 2 * s) + eb * (2 + s) + nb * s
@@ -237,7 +238,7 @@ def main():
         "data": "synthetic (seeded GP draw + noise; W from the device kriging build)",
         "config": {
             "workload": "dual-res 1mm: highres 128x160x128 @1mm + stdres 64x80x64 @2mm, embedding 256x320x256, "
-                        "L=25, neighbourhood 6.9mm, tau2=0.806 psi=0.132 nu=1.5, state mode compat",
+                        "L=25, thin 3, neighbourhood 6.9mm, tau2=0.806 psi=0.132 nu=1.5, state mode compat",
             "n_h": int(p["Yh"].size), "n_s": int(p["Ys"].size), "W_nnz": int(p["W"].nnz),
             "kriging_unique_solves": int(p["kriging_unique"]), "kriging_build_s": round(p["kriging_s"], 2),
             "W_layout": wst,

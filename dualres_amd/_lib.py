@@ -75,6 +75,10 @@ SIGNATURES = {
     "dr_w_stats": [_P, _I64],
     "dr_set_output_indices": [_P, _I32, C.c_int64],
     "dr_get_mu_output": [_P, _D, C.c_int64],
+    "dr_estimate_bytes": [C.POINTER(dr_config), _I32, C.c_int64, C.c_int64, C.c_int64, _I64],
+    "dr_summary_reset": [_P],
+    "dr_summary_update": [_P],
+    "dr_summary_read": [_P, _D, _D, _D, C.c_int64, _I64],
     "dr_get_mu_data": [_P, _D, C.c_int64],
     "dr_set_sigma_sq_inv": [_P, _D],
     "dr_get_sigma_sq_inv": [_P, _D],

@@ -140,6 +140,12 @@ struct ChainBase {
   virtual void add_data_to_mu() = 0;
   virtual void set_region(const int r[3]) = 0;
   virtual void get_mu_region_gather(const std::vector<int32_t>& gidx, double* out) = 0;
+  virtual void get_mu_out(double* out, long n) = 0;
+  virtual void set_outputs() = 0;
+  virtual void summary_reset() = 0;
+  virtual void summary_update() = 0;
+  virtual void summary_read(double* s1, double* s2, double* sa, long n, int64_t* count) = 0;
+  long n_out_dev = 0;  // output voxels held on the device
   virtual void update(double eps, int L, double u, const double* zr, const double* zi, dr_update_result* res) = 0;
   virtual void leapfrog(double eps, int L, dr_update_result* res) = 0;
   virtual void set_state(const double* mr, const double* mi, const double* pr, const double* pi) = 0;
@@ -175,6 +181,10 @@ struct Chain : public ChainBase {
   DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
   DevBuf<T> Wvals, WTvals, Ysd, vsd, outbuf;
   DevBuf<double> part, sums;
+  // posterior summary at the output voxels (mcmc_summary), float64
+  DevBuf<double> sm1, sm2, sma;
+  DevBuf<unsigned long long> smax;
+  int64_t nsum = 0;
   // stencil + boundary W (see build_stencil)
   struct WStencil {
     DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_dl, vg, vub, wcls;
@@ -187,7 +197,6 @@ struct Chain : public ChainBase {
   } ws;
   long pstride = 0;
   int slot_nblk[NSLOT] = {0};
-  std::vector<int32_t> out_g;  // region grid index of each output voxel
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // per-kernel profile of one leapfrog step: events between launches
   static constexpr int kMarks = 12;
@@ -223,7 +232,7 @@ struct Chain : public ChainBase {
     const AxisFFT<U> ad = axis_dev<U>(a);
     DR_REQUIRE(k_in >= 1 && k_in <= ad.M && k_out >= 1 && k_out <= ad.M && NB >= 1 && n_outer >= 1,
                DR_ELOGIC, "strided pass: bad extents");
-    const int tile = sizeof(U) == 4 ? kTileElems : kTileElems;
+    const int tile = kTileStrided;
     const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, tile);
     const int TB = 1 << tbs;
     dim3 grid(cdiv(NB, TB), n_outer);
@@ -238,7 +247,7 @@ struct Chain : public ChainBase {
   void xr2c(int nrows, Pro pro, Epi epi, int slot = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileElems);
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileX);
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -251,7 +260,7 @@ struct Chain : public ChainBase {
   void xc2r(int nrows, Pro pro, Epi epi, int slot = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileElems);
+    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileX);
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -460,12 +469,75 @@ struct Chain : public ChainBase {
     DR_HIP(hipMemcpyAsync(gI.p, g.data(), n_h * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     DR_HIP(hipStreamSynchronize(stream));
     if (data_set) upload_data();
-    if (!out_ijk.empty()) {
-      out_g.resize(out_ijk.size() / 3);
-      for (size_t i = 0; i < out_g.size(); ++i)
-        out_g[i] = (int32_t)region_index(out_ijk[3 * i], out_ijk[3 * i + 1], out_ijk[3 * i + 2]);
-    }
+    set_outputs();
     stats_valid = false;
+  }
+
+  // output voxels (the data voxels until set_output_indices) on the region
+  // grid, and the running summary over them, which this resets
+  void set_outputs() override {
+    std::vector<int32_t> og;
+    if (out_ijk.empty()) {
+      og.resize(n_h);
+      for (long i = 0; i < n_h; ++i) og[i] = (int32_t)region_index(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+    } else {
+      og.resize(out_ijk.size() / 3);
+      for (size_t i = 0; i < og.size(); ++i)
+        og[i] = (int32_t)region_index(out_ijk[3 * i], out_ijk[3 * i + 1], out_ijk[3 * i + 2]);
+    }
+    n_out_dev = (long)og.size();
+    const size_t no = std::max<size_t>(og.size(), 1);
+    outg.alloc(no);
+    outbuf.alloc(no);
+    sm1.alloc(no);
+    sm2.alloc(no);
+    sma.alloc(no);
+    if (!smax.p) smax.alloc(1);
+    if (!og.empty())
+      DR_HIP(hipMemcpyAsync(outg.p, og.data(), og.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    summary_reset();
+    DR_HIP(hipStreamSynchronize(stream));
+  }
+
+  void get_mu_out(double* out, long n) override {
+    DR_REQUIRE(n == n_out_dev, DR_ELOGIC, "dr_get_mu_output: wrong output length");
+    refresh_stats();
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, stream, n, outg.p, RA.p, outbuf.p);
+    DR_HIP(hipGetLastError());
+    std::vector<T> h(n);
+    DR_HIP(hipMemcpyAsync(h.data(), outbuf.p, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipStreamSynchronize(stream));
+    for (long i = 0; i < n; ++i) out[i] = (double)h[i];
+  }
+  void summary_reset() override {
+    sm1.zero(stream);
+    sm2.zero(stream);
+    sma.zero(stream);
+    nsum = 0;
+  }
+  // mcmc_summary<T>::update (gaussian_process_model.h:205-224) of the current
+  // state at the output voxels
+  void summary_update() override {
+    refresh_stats();
+    const long n = n_out_dev;
+    if (n == 0) return;
+    const int g = std::min(cdiv(n, kThreads), 4096);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, smax.p);
+    hipLaunchKernelGGL(k_sum_gather<T>, dim3(g), dim3(kThreads), 0, stream, n, outg.p, RA.p, outbuf.p, smax.p);
+    hipLaunchKernelGGL(k_sum_acc<T>, dim3(g), dim3(kThreads), 0, stream, n, outbuf.p, smax.p, sm1.p, sm2.p, sma.p);
+    DR_HIP(hipGetLastError());
+    ++nsum;
+  }
+  void summary_read(double* s1, double* s2, double* sa, long n, int64_t* count) override {
+    DR_REQUIRE(n == n_out_dev, DR_ELOGIC, "dr_summary_read: wrong output length");
+    if (n > 0) {
+      if (s1) DR_HIP(hipMemcpyAsync(s1, sm1.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (s2) DR_HIP(hipMemcpyAsync(s2, sm2.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (sa) DR_HIP(hipMemcpyAsync(sa, sma.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    DR_HIP(hipStreamSynchronize(stream));
+    if (count) *count = nsum;
   }
 
   // ------------------------------------------------------------ class-stencil W
@@ -533,6 +605,24 @@ struct Chain : public ChainBase {
   // k_wtv_bnd).  cols: image-region grid index of every entry.  Returns false
   // (plain CSR kernels) when a boundary row spans more than 16 voxels on an
   // axis or has no free anchor position.
+  // Entry order for the quad loads of k_wmu_bnd / k_wtv_bnd: a list
+  // [start, end) (a multiple of 4) is cut into blocks of `lanes` quads; in a
+  // block of nq quads, quad l holds entries l, l + nq, l + 2 nq, l + 3 nq, so
+  // the k-th gather of all lanes reads consecutive entries.
+  static void interleave_quads(std::vector<uint16_t>& d, std::vector<T>& w, long start, long end, int lanes) {
+    std::vector<uint16_t> td;
+    std::vector<T> tw;
+    for (long b0 = start; b0 < end; b0 += 4L * lanes) {
+      const long nb = std::min<long>(4L * lanes, end - b0), nq = nb / 4;
+      td.assign(d.begin() + b0, d.begin() + b0 + nb);
+      tw.assign(w.begin() + b0, w.begin() + b0 + nb);
+      for (long l = 0; l < nq; ++l)
+        for (int k = 0; k < 4; ++k) {
+          d[b0 + 4 * l + k] = td[l + k * nq];
+          w[b0 + 4 * l + k] = tw[l + k * nq];
+        }
+    }
+  }
   static long fdiv(long a, long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
   static long fmod_(long a, long b) { return a - b * fdiv(a, b); }
   bool build_stencil(const std::vector<int32_t>& rp, const std::vector<int32_t>& cols, const std::vector<T>& vals) {
@@ -619,23 +709,24 @@ struct Chain : public ChainBase {
       }
     }
     const int nD = (int)D.size();
-    long lo[3], hi[3], L[3];
+    long lo[3], hi[3], L[3], omin[3], omax[3];
     for (int a = 0; a < 3; ++a) {
       const long ubmin = fdiv(-c[a], sg[a]), ubmax = fdiv(reg[a] - 1 - c[a], sg[a]);
-      long omin = 0, omax = 0;
+      omin[a] = 0;
+      omax[a] = 0;
       for (int s = 0; s < nD; ++s) {
         const long o = -fdiv(d3[3 * s + a], sg[a]);
-        omin = std::min(omin, o);
-        omax = std::max(omax, o);
+        omin[a] = std::min(omin[a], o);
+        omax[a] = std::max(omax[a], o);
       }
-      lo[a] = ubmin + omin;
-      hi[a] = ubmax + omax;
+      lo[a] = ubmin + omin[a];
+      hi[a] = ubmax + omax[a];
       L[a] = hi[a] - lo[a] + 1;
     }
-    if ((long)D.size() * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of k_wmu_int
     const long nVl = L[0] * L[1] * L[2];
     DR_REQUIRE(nVl < (1L << 31), DR_EDOMAIN, "W stencil lattice too large");
     auto flatL = [&](const long* u) { return (u[0] - lo[0]) + L[0] * ((u[1] - lo[1]) + L[1] * (u[2] - lo[2])); };
+    if ((long)nD * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of k_wmu_int
     const int ncls = (int)(sg[0] * sg[1] * sg[2]);
     std::vector<int32_t> sub_start(ncls + 1, 0), sub_dl(nD);
     std::vector<T> sub_w(nD);
@@ -699,7 +790,6 @@ struct Chain : public ChainBase {
       vub.push_back((int32_t)safe_ub);
       if (vg.size() % 64 == 1) wcls.push_back(0);
     }
-    if (wcls.size() * 64 < vg.size()) wcls.resize(vg.size() / 64, 0);
     // 4. boundary rows: packed offsets from the row's bounding-box corner and
     //    a distinct anchor position (one of the row's own entries) for v
     std::vector<int32_t> brow, brp{0}, bbase, banchor;
@@ -742,6 +832,7 @@ struct Chain : public ChainBase {
         bd.push_back(0);
         bw.push_back((T)0);
       }
+      interleave_quads(bd, bw, brp.back(), (long)bw.size(), 64);
       brow.push_back((int32_t)i);
       bbase.push_back((int32_t)base);
       banchor.push_back(an);
@@ -777,6 +868,7 @@ struct Chain : public ChainBase {
         }
       }
     }
+    for (long d = 0; d < n_h; ++d) interleave_quads(tdall, twall, tstart[d], tstart[d + 1], 16);
     std::vector<int32_t> tg, trp{0};
     for (long d = 0; d < n_h; ++d)
       if (tstart[d + 1] > tstart[d]) {
@@ -795,14 +887,13 @@ struct Chain : public ChainBase {
         DR_REQUIRE((long)ibase[k] + fmin >= 0 && (long)ibase[k] + fmax < nreg, DR_ELOGIC,
                    "W stencil: interior entry outside the region");
       }
-    }
-    {
       std::vector<long> smin(ncls, LONG_MAX), smax(ncls, LONG_MIN);
       for (int k = 0; k < ncls; ++k)
         for (int t = sub_start[k]; t < sub_start[k + 1]; ++t) {
           smin[k] = std::min(smin[k], (long)sub_dl[t]);
           smax[k] = std::max(smax[k], (long)sub_dl[t]);
         }
+      DR_REQUIRE(wcls.size() * 64 == vg.size() && vg.size() % kThreads == 0, DR_ELOGIC, "W stencil: voxel padding");
       for (size_t t = 0; t < vg.size(); ++t) {
         const int k = wcls[t / 64];
         DR_REQUIRE(k >= 0 && k < ncls, DR_ELOGIC, "W stencil: residue class out of range");
@@ -976,8 +1067,9 @@ struct Chain : public ChainBase {
       double* pp = slot >= 0 ? pslot(slot) : nullptr;
       const int R0 = reg[0], R01 = reg[0] * reg[1];
       if (ws.n_int > 0) {
-        hipLaunchKernelGGL(k_wmu_int<T>, dim3(gi), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream, ws.n_int, ws.irow.p, ws.ibase.p,
-                           ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, R, Ysd.p, (T)sigma[1], ws.Vl.p, pp);
+        hipLaunchKernelGGL(k_wmu_int<T>, dim3(gi), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream,
+                           ws.n_int, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, R, Ysd.p,
+                           (T)sigma[1], ws.Vl.p, pp);
         DR_HIP(hipGetLastError());
       }
       mark();
@@ -1030,7 +1122,7 @@ struct Chain : public ChainBase {
   void zfused(MidLeapfrog<T> mid, int mid_slot) {
     const AxisFFT<T>& ad = axis_dev<T>(2);
     const int NB = M[1] * H0;
-    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, NB, kTileElems);
+    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, NB, kTileStrided);
     const int TB = 1 << tbs;
     const int grid = cdiv(NB, TB);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<T>);
@@ -1043,7 +1135,7 @@ struct Chain : public ChainBase {
   }
   int zgrid() const {
     const int NB = M[1] * H0;
-    return cdiv(NB, 1 << pick_tbs(M[2], M[2], sizeof(cx<T>), false, NB, kTileElems));
+    return cdiv(NB, 1 << pick_tbs(M[2], M[2], sizeof(cx<T>), false, NB, kTileStrided));
   }
 
   // ------------------------------------------------------------ HMC iteration
@@ -1332,6 +1424,48 @@ int dr_device_count(int* n) {
   return guarded([&] { DR_HIP(hipGetDeviceCount(n)); });
 }
 
+// torus dims: the reference rule 2^ceil(log2(2 max_ijk)) (MultiResParameters.h:191-207)
+// or the caller's even embedding
+static void embedding_dims(const dr_config* cfg, const int mx[3], int M[3]) {
+  const bool rule = cfg->embed[0] == 0 && cfg->embed[1] == 0 && cfg->embed[2] == 0;
+  for (int a = 0; a < 3; ++a) {
+    if (rule) {
+      const int d = 2 * mx[a];
+      DR_REQUIRE(d >= 2, DR_ELOGIC, "circulant_base_3d: Cannot have dimensions <= 1");
+      M[a] = (int)std::pow(2.0, std::ceil(std::log((double)d) / std::log(2.0)));
+    } else {
+      M[a] = cfg->embed[a];
+      DR_REQUIRE(M[a] >= 2 && M[a] % 2 == 0 && M[a] >= 2 * mx[a], DR_EDOMAIN,
+                 "embedding must be even and >= 2*max_ijk along every axis");
+    }
+    DR_REQUIRE(M[a] <= 2048, DR_EDOMAIN, "embedding axis longer than 2048");
+  }
+}
+
+int dr_estimate_bytes(const dr_config* cfg, const int32_t* ijk, int64_t n_h, int64_t n_s, int64_t w_nnz,
+                      int64_t* bytes) {
+  return guarded([&] {
+    DR_REQUIRE(cfg && ijk && bytes && n_h > 0, DR_ELOGIC, "dr_estimate_bytes: null argument or empty data");
+    int mx[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n_h; ++i)
+      for (int a = 0; a < 3; ++a) mx[a] = std::max(mx[a], ijk[3 * i + a]);
+    int M[3];
+    embedding_dims(cfg, mx, M);
+    const int64_t s = cfg->precision == DR_FP64 ? 8 : 4, cb = 2 * s;
+    const int64_t H0 = M[0] / 2 + 1, Nh = H0 * M[1] * M[2];
+    const int64_t nreg = (int64_t)(mx[0] + 1) * (mx[1] + 1) * (mx[2] + 1);
+    int64_t b = Nh * (5 * cb + s + 8);                                   // mu A/B, p, T1, T2, 1/lambda, lambda
+    if (cfg->state_mode == DR_STATE_COMPAT) b += 3 * Nh * cb;            // imaginary chain
+    int64_t ps = std::max<int64_t>({1024, Nh / 8 + 1, (int64_t)M[1] * M[2] + 1, (int64_t)M[1] * H0 + 1});
+    if (n_s > 0) ps = std::max<int64_t>(ps, (n_s + 3) / 4 + n_s + 2);
+    b += 16 * ps * 8;                                                    // reduction partials
+    b += nreg * (4 * s + 1) + n_h * (8 + s + 24);                        // region fields, indices, summary
+    if (n_s > 0) b += 2 * w_nnz * (4 + s) + n_s * (8 + 2 * s) + nreg * s; // W (CSR bound), Ys, v
+    b += 2 * Nh * 16;                                                    // spectrum set-up scratch
+    *bytes = b;
+  });
+}
+
 int dr_create(const dr_config* cfg, const int32_t* ijk, int64_t n_h, dr_chain** out) {
   return guarded([&] {
     DR_REQUIRE(cfg && ijk && out && n_h > 0, DR_ELOGIC, "dr_create: null argument or empty data");
@@ -1356,19 +1490,7 @@ int dr_create(const dr_config* cfg, const int32_t* ijk, int64_t n_h, dr_chain** 
         DR_REQUIRE(ijk[3 * i + a] >= 0, DR_EDOMAIN, "negative data index");
         mx[a] = std::max(mx[a], ijk[3 * i + a]);
       }
-    const bool rule = cfg->embed[0] == 0 && cfg->embed[1] == 0 && cfg->embed[2] == 0;
-    for (int a = 0; a < 3; ++a) {
-      if (rule) {
-        const int d = 2 * mx[a];
-        DR_REQUIRE(d >= 2, DR_ELOGIC, "circulant_base_3d: Cannot have dimensions <= 1");
-        c->M[a] = (int)std::pow(2.0, std::ceil(std::log((double)d) / std::log(2.0)));
-      } else {
-        c->M[a] = cfg->embed[a];
-        DR_REQUIRE(c->M[a] >= 2 && c->M[a] % 2 == 0 && c->M[a] >= 2 * mx[a], DR_EDOMAIN,
-                   "embedding must be even and >= 2*max_ijk along every axis");
-      }
-      DR_REQUIRE(c->M[a] <= 2048, DR_EDOMAIN, "embedding axis longer than 2048");
-    }
+    embedding_dims(cfg, mx, c->M);
     c->build();
     *out = new dr_chain{std::move(c)};
   });
@@ -1444,27 +1566,24 @@ int dr_set_output_indices(dr_chain* c, const int32_t* ijk, int64_t n_out) {
     int r[3];
     for (int a = 0; a < 3; ++a) r[a] = std::max(k.reg[a], mx[a] + 1);
     k.set_region(r);
-    const int rr[3] = {k.reg[0], k.reg[1], k.reg[2]};
-    (void)rr;
+    k.set_outputs();
   });
 }
 
 int dr_get_mu_output(dr_chain* c, double* out, int64_t n_out) {
-  return guarded([&] {
-    auto& k = *c->impl;
-    std::vector<int32_t> g;
-    if (k.out_ijk.empty()) {
-      g.resize(k.n_h);
-      for (int64_t i = 0; i < k.n_h; ++i)
-        g[i] = (int32_t)k.region_index(k.ijk[3 * i], k.ijk[3 * i + 1], k.ijk[3 * i + 2]);
-    } else {
-      g.resize(k.out_ijk.size() / 3);
-      for (size_t i = 0; i < g.size(); ++i)
-        g[i] = (int32_t)k.region_index(k.out_ijk[3 * i], k.out_ijk[3 * i + 1], k.out_ijk[3 * i + 2]);
-    }
-    DR_REQUIRE((int64_t)g.size() == n_out, DR_ELOGIC, "dr_get_mu_output: wrong output length");
-    k.get_mu_region_gather(g, out);
-  });
+  return guarded([&] { c->impl->get_mu_out(out, (long)n_out); });
+}
+
+int dr_summary_reset(dr_chain* c) {
+  return guarded([&] { c->impl->summary_reset(); });
+}
+
+int dr_summary_update(dr_chain* c) {
+  return guarded([&] { c->impl->summary_update(); });
+}
+
+int dr_summary_read(dr_chain* c, double* sum_mu, double* sum_mu2, double* sum_abs, int64_t n_out, int64_t* n_updates) {
+  return guarded([&] { c->impl->summary_read(sum_mu, sum_mu2, sum_abs, (long)n_out, n_updates); });
 }
 
 int dr_get_mu_data(dr_chain* c, double* out, int64_t n_h) {

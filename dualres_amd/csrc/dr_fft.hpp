@@ -29,7 +29,13 @@
 namespace drh {
 
 constexpr int kMaxStages = 16;
-constexpr int kTileElems = 4096;  // TB * M per workgroup (16 complex per lane)
+// complex elements per lane per stage, and the tile (TB * M per workgroup):
+// strided (y/z) passes batch 16, the x passes 8 (more workgroups in flight
+// for their short rows)
+constexpr int kEptStrided = 16;
+constexpr int kEptX = 8;
+constexpr int kTileStrided = kEptStrided * kThreads;
+constexpr int kTileX = kEptX * kThreads;
 
 template <typename T>
 struct AxisFFT {
@@ -147,13 +153,13 @@ template <typename T, bool INV> struct Dft<T, 7, INV> : DftOdd<T, 7, INV> {};
 // LDS holds TB = 2^tbs lines; element k of line b sits at lds[k*P + b].
 // Stage with span Ns: butterfly j reads j + r*M/R, twiddles by
 // w^(r * (j mod Ns) * M/(Ns R)), writes (j - j mod Ns) R + j mod Ns + r Ns.
-template <typename T, int R, bool INV>
+template <typename T, int R, bool INV, int EPT>
 __device__ __forceinline__ void fft_stage(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax, int Ns) {
   const int TB = 1 << tbs;
   const int BF = ax.M / R;
   const int total = BF << tbs;
   const int estep = (ax.M / (Ns * R)) * ax.tws;
-  constexpr int IT = (16 + R - 1) / R;
+  constexpr int IT = (EPT + R - 1) / R;
   cx<T> v[IT][R];
   int dst[IT];
 #pragma unroll
@@ -189,18 +195,18 @@ __device__ __forceinline__ void fft_stage(cx<T>* lds, int P, int tbs, const Axis
   __syncthreads();
 }
 
-template <typename T, bool INV>
+template <typename T, bool INV, int EPT>
 __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax) {
   int Ns = 1;
   for (int s = 0; s < ax.nst; ++s) {
     const int R = ax.radix[s];
     switch (R) {
-      case 8: fft_stage<T, 8, INV>(lds, P, tbs, ax, Ns); break;
-      case 4: fft_stage<T, 4, INV>(lds, P, tbs, ax, Ns); break;
-      case 2: fft_stage<T, 2, INV>(lds, P, tbs, ax, Ns); break;
-      case 3: fft_stage<T, 3, INV>(lds, P, tbs, ax, Ns); break;
-      case 5: fft_stage<T, 5, INV>(lds, P, tbs, ax, Ns); break;
-      case 7: fft_stage<T, 7, INV>(lds, P, tbs, ax, Ns); break;
+      case 8: fft_stage<T, 8, INV, EPT>(lds, P, tbs, ax, Ns); break;
+      case 4: fft_stage<T, 4, INV, EPT>(lds, P, tbs, ax, Ns); break;
+      case 2: fft_stage<T, 2, INV, EPT>(lds, P, tbs, ax, Ns); break;
+      case 3: fft_stage<T, 3, INV, EPT>(lds, P, tbs, ax, Ns); break;
+      case 5: fft_stage<T, 5, INV, EPT>(lds, P, tbs, ax, Ns); break;
+      case 7: fft_stage<T, 7, INV, EPT>(lds, P, tbs, ax, Ns); break;
       default: break;
     }
     Ns *= R;
@@ -214,7 +220,7 @@ __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFF
 //   c2r_x:    cx<T> Pro::load(int row, int k);   void Epi::store2(int row, int n, cx<T> z)  (samples 2n, 2n+1)
 // and void finish(double* red, int block) on both for optional reductions.
 
-template <typename T, bool INV, class Pro, class Epi>
+template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided>
 __global__ void __launch_bounds__(kThreads)
 k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -231,7 +237,7 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
     lds[k * TB + b] = v;
   }
   __syncthreads();
-  fft_lds<T, INV>(lds, TB, tbs, ax);
+  fft_lds<T, INV, EPT>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b]);
@@ -245,7 +251,7 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
 // the leapfrog kick and drift in the spectral domain sit between the forward
 // z pass of one gradient and the inverse z pass of the next.
 //   cx<T> Mid::apply(int k, int b, cx<T> v)  for every k < M.
-template <typename T, class Pro, class Mid, class Epi>
+template <typename T, class Pro, class Mid, class Epi, int EPT = kEptStrided>
 __global__ void __launch_bounds__(kThreads)
 k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -261,13 +267,13 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
     lds[k * TB + b] = v;
   }
   __syncthreads();
-  fft_lds<T, false>(lds, TB, tbs, ax);
+  fft_lds<T, false, EPT>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) lds[k * TB + b] = mid.apply(k, b0 + b, lds[k * TB + b]);
   }
   __syncthreads();
-  fft_lds<T, true>(lds, TB, tbs, ax);
+  fft_lds<T, true, EPT>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) epi.store(0, k, b0 + b, lds[k * TB + b]);
@@ -281,7 +287,7 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
 // Real rows of length M0 (first n_in samples nonzero) -> M0/2+1 complex.
 // ax describes the complex FFT of length Mh = M0/2 with tws = 2 into the
 // length-M0 table, so ax.tw[k] = exp(-2 pi i k / M0).
-template <typename T, class Pro, class Epi>
+template <typename T, class Pro, class Epi, int EPT = kEptX>
 __global__ void __launch_bounds__(kThreads)
 k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -299,7 +305,7 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
     lds[n * P + l] = v;
   }
   __syncthreads();
-  fft_lds<T, false>(lds, P, tbs, ax);
+  fft_lds<T, false, EPT>(lds, P, tbs, ax);
   const int H = Mh + 1;
   for (int idx = threadIdx.x; idx < TB * H; idx += kThreads) {
     const int l = idx / H, k = idx - l * H;
@@ -321,7 +327,7 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
 
 // M0/2+1 complex per row -> real rows; the epilogue receives packed pairs
 // (x[2n], x[2n+1]) = (Re z[n], Im z[n]) unscaled.
-template <typename T, class Pro, class Epi>
+template <typename T, class Pro, class Epi, int EPT = kEptX>
 __global__ void __launch_bounds__(kThreads)
 k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -357,7 +363,7 @@ k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
     lds[k * P + l] = zk;
   }
   __syncthreads();
-  fft_lds<T, true>(lds, P, tbs, ax);
+  fft_lds<T, true, EPT>(lds, P, tbs, ax);
   for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
     const int l = idx / Mh, n = idx - l * Mh;
     const int row = row0 + l;

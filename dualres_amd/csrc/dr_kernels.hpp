@@ -378,9 +378,10 @@ __device__ __forceinline__ unsigned lo16(unsigned x) { return x & 0xffffu; }
 __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 
 // interior rows: 8 lanes per row, lane q takes stencil entries q, q+8, ...
-// (consecutive lanes read consecutive entries of an x-run), then a fixed
-// butterfly over the 8 lanes
-// (stencil table staged in LDS: nD * (4 + sizeof(T)) bytes of dynamic LDS)
+// (consecutive lanes read consecutive entries of an x-run) from the stencil
+// table staged in LDS (nD * (4 + sizeof(T)) bytes), issuing batches of 8
+// independent gathers before their multiply-adds; the 8 partial sums are
+// combined by a fixed butterfly
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_wmu_int(int n, const int* __restrict__ row, const int* __restrict__ base, const int* __restrict__ lidx, int nD,
@@ -400,8 +401,16 @@ k_wmu_int(int n, const int* __restrict__ row, const int* __restrict__ base, cons
   T acc = (T)0;
   if (k < n) {
     const T* Rb = R + base[k];
-#pragma unroll 4
-    for (int s = q; s < nD; s += 8) acc += sw[s] * Rb[so[s]];
+    constexpr int B = 8;
+    int s = q;
+    for (; s + 8 * (B - 1) < nD; s += 8 * B) {
+      T v[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) v[b] = Rb[so[s + 8 * b]];
+#pragma unroll
+      for (int b = 0; b < B; ++b) acc += sw[s + 8 * b] * v[b];
+    }
+    for (; s < nD; s += 8) acc += sw[s] * Rb[so[s]];
   }
   acc += __shfl_xor(acc, 1, 8);
   acc += __shfl_xor(acc, 2, 8);
@@ -453,8 +462,9 @@ k_wmu_bnd(int n, const int* __restrict__ row, const int* __restrict__ rp, const 
 
 // W^T v of the interior rows at every data voxel (written, not added).  The
 // voxel list is bucketed by residue class and each bucket padded to whole
-// wavefronts, so a wavefront reads one sub-stencil with scalar loads; padding
-// lanes (g < 0) compute on a valid lattice base and store nothing.
+// wavefronts, so a wavefront reads one sub-stencil with scalar loads, in
+// batches of 16 independent gathers; padding lanes (g < 0) compute on a
+// valid lattice base and store nothing.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_wtv_int(const int* __restrict__ g, const int* __restrict__ ub, const int* __restrict__ wcls,
@@ -465,8 +475,16 @@ k_wtv_int(const int* __restrict__ g, const int* __restrict__ ub, const int* __re
   const int s0 = sub_start[cls], s1 = sub_start[cls + 1];
   const T* V = Vl + ub[t];
   T acc = (T)0;
-#pragma unroll 4
-  for (int s = s0; s < s1; ++s) acc += sub_w[s] * V[sub_dl[s]];
+  constexpr int B = 16;
+  int s = s0;
+  for (; s + B <= s1; s += B) {
+    T v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) v[b] = V[sub_dl[s + b]];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc += sub_w[s + b] * v[b];
+  }
+  for (; s < s1; ++s) acc += sub_w[s] * V[sub_dl[s]];
   const int gj = g[t];
   if (gj >= 0) out[gj] = acc;
 }
@@ -500,6 +518,47 @@ k_wtv_bnd(int n, const int* __restrict__ g, const int* __restrict__ rp, const ui
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
   if (k < n && lane == 0) out[gj] += acc;
+}
+
+// mcmc_summary<T>::update (gaussian_process_model.h:205-224) at the output
+// voxels with float64 accumulators.  Pass 1 gathers mu and the largest |mu|
+// (atomicMax on the bit pattern of a non-negative double, which orders like
+// its value); pass 2 adds mu, mu^2 and |mu| / max(max |mu|, 1e-6).
+static __global__ void __launch_bounds__(64) k_zero_u64(unsigned long long* p) {
+  if (threadIdx.x == 0) p[threadIdx.x] = 0ull;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_sum_gather(long n, const int* __restrict__ g, const T* __restrict__ R, T* __restrict__ out,
+             unsigned long long* amax) {
+  __shared__ double red[kThreads];
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+    const T v = R[g[i]];
+    out[i] = v;
+    m = fmax(m, fabs((double)v));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(amax, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_sum_acc(long n, const T* __restrict__ v, const unsigned long long* __restrict__ amax, double* __restrict__ s1,
+          double* __restrict__ s2, double* __restrict__ sa) {
+  const double M = fmax(__longlong_as_double((long long)amax[0]), 1e-6);
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+    const double x = (double)v[i];
+    s1[i] += x;
+    s2[i] += x * x;
+    sa[i] += fabs(x) / M;
+  }
 }
 
 // sum over the half spectrum of w |X|^2 a(k) with a = invlam (prior) or

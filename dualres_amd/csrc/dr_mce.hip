@@ -122,7 +122,7 @@ struct Fft3d {
   template <bool INV, class Pro, class Epi>
   void strided(int a, int n_outer, int NB, Pro pro, Epi epi) {
     const AxisFFT<double>& ad = ax[a];
-    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<double>), false, NB, kTileElems);
+    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<double>), false, NB, kTileStrided);
     const int TB = 1 << tbs;
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<double>);
     hipLaunchKernelGGL((k_strided<double, INV, Pro, Epi>), dim3(cdiv(NB, TB), n_outer), dim3(kThreads), lds, s, ad,
@@ -132,7 +132,7 @@ struct Fft3d {
   int xtbs(int nrows) const {
     const int Mh = ax[0].M;
     const int lines = std::max(std::max(1, nrows / 2048), cdiv(1024, std::max(Mh, 1)));
-    return pick_tbs(Mh, Mh + 1, sizeof(cx<double>), true, lines, kTileElems);
+    return pick_tbs(Mh, Mh + 1, sizeof(cx<double>), true, lines, kTileX);
   }
   template <class Pro>
   void r2c(Pro pro, cx<double>* X) {

@@ -119,6 +119,20 @@ class Chain:
         L.check(self._lib.dr_get_mu_output(self._h, L.dptr(out), self.n_out))
         return out
 
+    def summary_reset(self):
+        L.check(self._lib.dr_summary_reset(self._h))
+
+    def summary_update(self):
+        """Add the current state at the output voxels to the device running sums."""
+        L.check(self._lib.dr_summary_update(self._h))
+
+    def summary_read(self):
+        """(sum mu, sum mu^2, sum |mu| / max|mu|, number of updates) at the output voxels."""
+        s1, s2, sa = np.empty(self.n_out), np.empty(self.n_out), np.empty(self.n_out)
+        n = C.c_int64()
+        L.check(self._lib.dr_summary_read(self._h, L.dptr(s1), L.dptr(s2), L.dptr(sa), self.n_out, C.byref(n)))
+        return s1, s2, sa, n.value
+
     def mu(self):
         out = np.empty(self.n_h)
         L.check(self._lib.dr_get_mu_data(self._h, L.dptr(out), self.n_h))

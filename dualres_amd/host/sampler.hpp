@@ -21,10 +21,20 @@
 
 namespace drhost {
 
+// the reference's global generator (defines.h:78); one per thread, so fits
+// running side by side on worker threads draw independent sequences
 inline std::mt19937& rng() {
-  static std::mt19937 g(42);  // defines.h:78
+  thread_local std::mt19937 g(42);
   return g;
 }
+
+// a stream that discards what is written (quiet fits)
+struct NullStream : std::ostream {
+  struct Buf : std::streambuf {
+    int overflow(int c) override { return c; }
+  } buf;
+  NullStream() : std::ostream(&buf) {}
+};
 
 template <typename RealType>
 class HMCParameters {
@@ -100,26 +110,25 @@ class HMCParameters {
          mh_sum_ = 0, mh_target_ = 0.65, t0_ = 10, kappa_ = 0.75;
 };
 
-// running moments (gaussian_process_model.h:191-302), accumulated in T
+// running moments (gaussian_process_model.h:191-302).  The moments of mu are
+// accumulated on the device in float64 (dr_summary_update) and copied out by
+// finish(); sigma and the log-likelihood sums stay on the host.
 template <typename T>
 struct McmcSummary {
-  std::vector<T> m1, m1abs, m2;
+  std::vector<double> m1, m1abs, m2;
   std::vector<T> sigma;
   T log_lik = 0, mh_rate = 0, sampling_time = 0;
   int samples_ = 0;
   McmcSummary(long n, int ns) : m1(n, 0), m1abs(n, 0), m2(n, 0), sigma(ns, 0) {}
-  void update(const std::vector<double>& mu, const std::vector<T>& s, T ll) {
-    T mx = (T)1e-6;
-    for (double v : mu) mx = std::max(mx, (T)std::fabs(v));
-    for (size_t i = 0; i < mu.size(); ++i) {
-      const T v = (T)mu[i];
-      m1[i] += v;
-      m1abs[i] += std::fabs(v) / mx;
-      m2[i] += v * v;
-    }
+  void update(Chain& chain, const std::vector<T>& s, T ll) {
+    check(dr_summary_update(chain.handle()));
     for (size_t i = 0; i < s.size(); ++i) sigma[i] += s[i];
     log_lik += ll;
     ++samples_;
+  }
+  void finish(Chain& chain) {
+    int64_t n = 0;
+    check(dr_summary_read(chain.handle(), m1.data(), m2.data(), m1abs.data(), (int64_t)m1.size(), &n));
   }
   int samples() const { return samples_ <= 0 ? 1 : samples_; }
   std::vector<double> mode_mu() const {
@@ -154,6 +163,7 @@ struct FitOptions {
   uint64_t seed = 1;
   bool monitor = false;
   bool output_samples = false;
+  bool quiet = false;  // no progress messages (served fits)
 };
 
 struct MultiResData {  // MultiResData.h:21-127
@@ -169,6 +179,8 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
                          HMCParameters<T>& hmc, std::ofstream& samples_out, const FitOptions& opt) {
   if (!same_orientation(hr, input_mask) || !same_orientation(hr, output_mask))
     throw std::domain_error("fit_model: image/mask oreintation error");
+  NullStream nullout;
+  std::ostream& out = opt.quiet ? static_cast<std::ostream&>(nullout) : std::cout;
   const BBox bb = bounding_box(input_mask);
   dr_config cfg{};
   cfg.n_datasets = data.n_datasets;
@@ -182,19 +194,19 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
   for (int i = 0; i < 16; ++i) cfg.qform[i] = hr.qto_xyz[i];
   cfg.seed = opt.seed;
   cfg.device = opt.device;
-  std::cout << "Selecting DFT algorithms... " << std::flush;
+  out << "Selecting DFT algorithms... " << std::flush;
   Chain chain(cfg, nonzero_indices_bounded_by_box(hr, bb));
-  std::cout << "Done!" << std::endl;
+  out << "Done!" << std::endl;
   int dims[3];
   int64_t N, nonpos;
   double maxeig;
   chain.spectrum(dims, &N, &nonpos, &maxeig);
-  std::cout << "Computing eigen values... Done!\n\t<Maximal eigenvalue = " << maxeig << ">\n"
+  out << "Computing eigen values... Done!\n\t<Maximal eigenvalue = " << maxeig << ">\n"
             << std::setprecision(2) << std::fixed << (1.0 - (double)nonpos / N) * 100
             << "% of eigen values have real-part > 0  (" << (N - nonpos) << "/" << N << ")" << std::endl;
-  std::cout.unsetf(std::ios::fixed);
-  std::cout << std::setprecision(6);
-  if (nonpos > 0) std::cout << "\t<Covariance function is numerically not positive definite>" << std::endl;
+  out.unsetf(std::ios::fixed);
+  out << std::setprecision(6);
+  if (nonpos > 0) out << "\t<Covariance function is numerically not positive definite>" << std::endl;
   chain.set_data(data.Yh, &data.Ys, &data.W);
   // _initialize_sigma (MultiResParameters.h:764-778), after the mu draw
   std::vector<double> sig(data.n_datasets);
@@ -209,9 +221,9 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
   chain.set_output_indices(nonzero_indices_bounded_by_box(output_mask, bb));
   McmcSummary<T> summ(chain.n_out(), data.n_datasets);
   int save_count = 0;
-  std::cout << "\nFitting model with rmHMC";
-  if (data.n_datasets > 1) std::cout << " and projected neighborhood process approximation";
-  std::cout << std::endl;
+  out << "\nFitting model with rmHMC";
+  if (data.n_datasets > 1) out << " and projected neighborhood process approximation";
+  out << std::endl;
   auto start = std::chrono::high_resolution_clock::now();
   std::uniform_real_distribution<double> U01(0, 1);
   std::vector<T> sigma(data.n_datasets);
@@ -221,7 +233,7 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
     const double u = U01(rng());
     const dr_update_result r = chain.update(eps, L, u);
     if (opt.monitor)
-      std::cout << std::setprecision(4) << "Energy Transition (" << r.H0 << " -> " << r.H1 << ");  step size ("
+      out << std::setprecision(4) << "Energy Transition (" << r.H0 << " -> " << r.H1 << ");  step size ("
                 << eps << ");  MH = (" << r.R << ")" << std::endl;
     // _update_sigma_sq_inv (MultiResParameters.h:820-859)
     {
@@ -236,13 +248,13 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
     hmc.update((T)r.mh_rate);
     if (hmc.save_iteration()) {
       if (save_count == 0) start = std::chrono::high_resolution_clock::now();
-      const std::vector<double> mu = chain.mu_();
       for (int i = 0; i < data.n_datasets; ++i) sigma[i] = (T)std::sqrt(1.0 / sig[i]);
       double ll = sig[0] * r.sse_h;
       if (data.n_datasets == 2) ll += sig[1] * r.sse_s;
       const T log_lik = (T)(-0.5 * ll);
-      summ.update(mu, sigma, log_lik);
+      summ.update(chain, sigma, log_lik);
       if (samples_out.is_open() && opt.output_samples) {
+        const std::vector<double> mu = chain.mu_();
         for (double v : mu) samples_out << (T)v << "\t";
         for (int i = 0; i < data.n_datasets; ++i) samples_out << sigma[i] << "\t";
         samples_out << (T)std::sqrt(data.theta[0]) << "\t" << log_lik;
@@ -252,6 +264,7 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
     }
   }
   const auto stop = std::chrono::high_resolution_clock::now();
+  summ.finish(chain);
   summ.sampling_time = (T)(std::chrono::duration_cast<std::chrono::milliseconds>(stop - start).count() / 1e3);
   T rate = hmc.metropolis_hastings_rate();
   if (std::isnan(rate)) rate = 0;

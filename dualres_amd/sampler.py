@@ -103,23 +103,53 @@ class HMCParameters:
 
 
 class McmcSummary:
-    """Running posterior moments (gaussian_process_model.h:191-302), float64."""
+    """Running posterior moments (gaussian_process_model.h:191-302), float64.
 
-    def __init__(self, n_mu, n_sigma):
+    With ``chain`` given the moments of mu are accumulated on the device
+    (``Chain.summary_update``) and ``update`` takes ``mu=None``; otherwise
+    they are accumulated here from the host vector ``mu``."""
+
+    def __init__(self, n_mu, n_sigma, chain=None):
         self.samples = 0
-        self.m1 = np.zeros(n_mu)
-        self.m1_abs = np.zeros(n_mu)
-        self.m2 = np.zeros(n_mu)
+        self.chain = chain
+        self._m = None
+        if chain is not None:
+            chain.summary_reset()
+        else:
+            self._m = (np.zeros(n_mu), np.zeros(n_mu), np.zeros(n_mu))
         self.sigma = np.zeros(n_sigma)
         self.log_lik = 0.0
 
     def update(self, mu, sigma, log_lik):
-        self.m1 += mu
-        self.m1_abs += np.abs(mu) / max(np.abs(mu).max(), 1e-6)
-        self.m2 += mu * mu
+        if self.chain is not None:
+            self.chain.summary_update()
+            self._m = None
+        else:
+            m1, m2, ma = self._m
+            m1 += mu
+            ma += np.abs(mu) / max(np.abs(mu).max(), 1e-6)
+            m2 += mu * mu
         self.sigma += sigma
         self.log_lik += log_lik
         self.samples += 1
+
+    def _moments(self):
+        if self._m is None:
+            s1, s2, sa, _ = self.chain.summary_read()
+            self._m = (s1, s2, sa)
+        return self._m
+
+    @property
+    def m1(self):
+        return self._moments()[0]
+
+    @property
+    def m2(self):
+        return self._moments()[1]
+
+    @property
+    def m1_abs(self):
+        return self._moments()[2]
 
     def _n(self):
         return max(self.samples, 1)
@@ -161,7 +191,7 @@ def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None):
     """
     rng = np.random.default_rng(seed)
     init_sigma(chain, rng, tau2)
-    summ = McmcSummary(chain.n_out, chain.n_datasets)
+    summ = McmcSummary(chain.n_out, chain.n_datasets, chain=chain)
     save_count = 0
     t_start = time.perf_counter()
     grads = 0
@@ -177,9 +207,8 @@ def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None):
             if save_count == 0:
                 t_start = time.perf_counter()
                 grads = 0
-            mu = chain.mu_()
             ll = -0.5 * (s[0] * r["sse_h"] + (s[1] * r["sse_s"] if chain.n_datasets == 2 else 0.0))
-            summ.update(mu, np.sqrt(1.0 / np.asarray(s)), ll)
+            summ.update(None, np.sqrt(1.0 / np.asarray(s)), ll)
             save_count += 1
         if on_iteration:
             on_iteration(hmc, r)

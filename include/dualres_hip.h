@@ -88,6 +88,12 @@ int dr_device_count(int* n);
  * (MultiResParameters.h:283-284) or the embedding is invalid. */
 int dr_create(const dr_config* cfg, const int32_t* ijk, int64_t n_h, dr_chain** out);
 int dr_destroy(dr_chain* c);
+/* Upper bound on the device bytes dr_create + dr_set_data allocate for this
+ * configuration (W counted in its CSR form), for admission control by a
+ * scheduler that shares one device between fits.  No counterpart in the
+ * reference. */
+int dr_estimate_bytes(const dr_config* cfg, const int32_t* ijk, int64_t n_h, int64_t n_s, int64_t w_nnz,
+                      int64_t* bytes);
 
 /* Spectrum report (MultiResParameters.h:258-281): embedding dims, N, the
  * number of eigenvalues with real part <= 0 and lambda[0]/N. */
@@ -111,8 +117,18 @@ int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, in
 int dr_w_layout(dr_chain* c, int force_csr);
 /* stats[0] = stencil layout active; [1] interior (stencil) rows; [2] stencil
  * entries; [3] boundary rows; [4] boundary entries; [5] nnz(W); [6] data
- * voxels touched by boundary rows; [7] padded voxel slots of the W^T pass. */
+ * voxels touched by boundary rows; [7] padded voxel slots of the interior W^T v pass. */
 int dr_w_stats(dr_chain* c, int64_t* stats);
+
+/* mcmc_summary<T> (gaussian_process_model.h:191-224) kept on the device:
+ * dr_summary_update adds mu, mu^2 and |mu| / max(max|mu|, 1e-6) of the
+ * current state at the output voxels to float64 running sums (the reference
+ * accumulates in T on the host).  set_output_indices and dr_summary_reset
+ * clear the sums; dr_summary_read copies them out (NULL skips an array). */
+int dr_summary_reset(dr_chain* c);
+int dr_summary_update(dr_chain* c);
+int dr_summary_read(dr_chain* c, double* sum_mu, double* sum_mu2, double* sum_abs, int64_t n_out,
+                    int64_t* n_updates);
 
 /* MultiResParameters::add_data_to_mu (:631-641). */
 int dr_add_data_to_mu(dr_chain* c);

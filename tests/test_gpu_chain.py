@@ -224,3 +224,32 @@ def test_w_stencil_layout_matches_csr(gpu_available, precision, dims):
     for k in ("H0", "H1"):
         assert abs(outs[1][1][k] - outs[0][1][k]) <= htol * abs(outs[0][1][k])
     assert abs(outs[1][2] - outs[0][2]) <= htol * abs(outs[0][2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_summary_matches_host_moments(gpu_available, precision):
+    """mcmc_summary::update on the device against the same moments built on
+    the host from mu_() (gaussian_process_model.h:205-224); both float64."""
+    from dualres_amd.sampler import McmcSummary
+    p = small_dual(dims=(16, 14, 12))
+    c = engine_chain(p, THETA2, precision=precision, state_mode="real")
+    c.add_data_to_mu()
+    out = p["ijk"][::3]
+    c.set_output_indices(out)
+    c.set_sigma_sq_inv([1.1, 0.6])
+    host = McmcSummary(c.n_out, 2)
+    dev = McmcSummary(c.n_out, 2, chain=c)
+    rng = np.random.default_rng(9)
+    for k in range(4):
+        z = (rng.standard_normal(c.N), rng.standard_normal(c.N))
+        c.update(0.05, 3, 0.3, noise=z)
+        host.update(c.mu_(), np.ones(2), -1.0)
+        dev.update(None, np.ones(2), -1.0)
+    for a, b in ((dev.m1, host.m1), (dev.m2, host.m2), (dev.m1_abs, host.m1_abs)):
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-300)
+    # var = (m2 - m1^2/n)/n cancels: the derived activation keeps ~1e-10 of the moments' 1e-14
+    np.testing.assert_allclose(dev.activation(), host.activation(), rtol=1e-9)
+    assert c.summary_read()[3] == 4
+    c.set_output_indices(out)          # resets the running sums
+    assert c.summary_read()[3] == 0 and not c.summary_read()[0].any()
+    c.close()

@@ -1,0 +1,70 @@
+"""The fit server (dualres_amd/host/server.hpp): whole fits served side by side
+on one device must give the same posterior moments, bit for bit, as the same
+requests served alone; admission keeps the device bytes under the budget."""
+import numpy as np
+import pytest
+
+from dualres_amd import problem, serve
+
+THETA = (0.806, 0.132, 1.5)
+
+
+def _requests():
+    s1 = problem.single_problem((16, 14, 12), 2.0, (1.0, 0.3, 1.5), seed_gp=11)
+    s2 = problem.single_problem((20, 16, 12), 2.0, (0.9, 0.2, 1.2), seed_gp=12)
+    d1 = problem.dual_problem((24, 20, 16), 1.0, THETA, seed_gp=13)
+    d2 = problem.dual_problem((20, 20, 16), 1.0, THETA, seed_gp=14)
+    kw = dict(burnin=20, nsave=15, thin=1, L=8)
+    return [
+        serve.make_request(s1["hr"], s1["q_hr"], (1.0, 0.3, 1.5), seed=1, precision="fp32", **kw),
+        serve.make_request(d1["hr"], d1["q_hr"], THETA, std=d1["std"], q_std=d1["q_std"], seed=2, precision="fp32",
+                           **kw),
+        serve.make_request(s2["hr"], s2["q_hr"], (0.9, 0.2, 1.2), seed=3, precision="fp64", **kw),
+        serve.make_request(d2["hr"], d2["q_hr"], THETA, std=d2["std"], q_std=d2["q_std"], seed=4, precision="fp64",
+                           state_mode="real", **kw),
+    ]
+
+
+@pytest.mark.gpu
+def test_results_identical_alone_and_in_mix(gpu_available):
+    reqs = _requests()
+    budget = 4 * max(serve.estimate_bytes(r) for r in reqs)
+    with serve.Server(budget, workers=3) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        mixed = [srv.wait(i, timeout=300) for i in ids]
+    with serve.Server(budget, workers=1) as srv:
+        alone = [srv.wait(srv.submit(r), timeout=300) for r in reqs]
+    for m, a, r in zip(mixed, alone, reqs):
+        assert m["status"] == 0 and a["status"] == 0, (m["error"], a["error"])
+        assert m["n_out"] == serve.n_outputs(r)
+        assert np.array_equal(m["mean"], a["mean"])
+        assert np.array_equal(m["var"], a["var"])
+        assert m["mh_rate"] == a["mh_rate"] and m["eps"] == a["eps"]
+        assert np.all(np.isfinite(m["mean"])) and np.all(m["var"] >= 0)
+        assert m["grad_evals"] == 15 * 9
+
+
+@pytest.mark.gpu
+def test_admission_keeps_budget(gpu_available):
+    reqs = [_requests()[1]] * 3
+    need = serve.estimate_bytes(reqs[0])
+    with serve.Server(int(1.5 * need), workers=3, backfill=False) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        res = [srv.wait(i, timeout=300) for i in ids]
+    assert all(r["status"] == 0 for r in res)
+    assert all(np.array_equal(r["mean"], res[0]["mean"]) for r in res)
+    # with room for one fit at a time the runs cannot overlap (the three
+    # submissions are ~simultaneous, so queue time ~ start time)
+    starts = [r["queue_s"] for r in res]
+    ends = [r["queue_s"] + r["run_s"] for r in res]
+    order = np.argsort(starts)
+    for a, b in zip(order[:-1], order[1:]):
+        assert starts[b] >= ends[a] - 0.05
+
+
+@pytest.mark.gpu
+def test_request_over_budget_is_refused(gpu_available):
+    r = _requests()[1]
+    with serve.Server(1024, workers=1) as srv:
+        out = srv.wait(srv.submit(r), timeout=30)
+    assert out["status"] == 5 and "budget" in out["error"]
