@@ -150,6 +150,81 @@ def profile_kernels(c, p, iters=3):
     return ks, n
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (calibrated against a plain device copy), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_dualres_1mm_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else int(k["traffic_bytes"])
+
+
+def calibration_copy(nbytes=256 * 2 ** 20, reps=3):
+    """Plain device-to-device copies of a known size through the HIP runtime;
+    their FETCH_SIZE / WRITE_SIZE per dispatch calibrate the HBM counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] calibration copy: {reps} x {nbytes} bytes device to device")
+
+
+def serving_leg(rates=(4.0, 16.0), n_req=24, workers=4, seed=20261202):
+    """Mixed fits through the fit server: Poisson arrivals at each rate, half
+    single- and half dual-resolution, highres 48^3 or 64x80x64 at 1.5 mm,
+    theta = (1, U(0.08, 0.2), U(0.8, 1.8)), 40 burn-in + 40 saved iterations of
+    L = 25 in FP32.  Returns p50/p99 submit-to-result latency per rate."""
+    from dualres_amd import problem, serve
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_req):
+        dims = (48, 48, 48) if i % 4 < 2 else (64, 80, 64)
+        th = (1.0, float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 1.8)))
+        kw = dict(burnin=40, nsave=40, thin=1, L=L_STEPS, seed=1000 + i, precision="fp32")
+        if i % 2 == 0:
+            q = problem.single_problem(dims, 1.5, th, seed_gp=100 + i)
+            reqs.append(serve.make_request(q["hr"], q["q_hr"], th, **kw))
+        else:
+            q = problem.dual_problem(dims, 1.5, th, seed_gp=100 + i)
+            reqs.append(serve.make_request(q["hr"], q["q_hr"], th, std=q["std"], q_std=q["q_std"], **kw))
+    out = []
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+        with serve.Server(200 * 10 ** 9, workers=workers) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for r, ta in zip(reqs, arrivals):
+                dt = ta - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(r))
+            res = [srv.wait(i, timeout=600) for i in ids]
+            wall = time.perf_counter() - t0
+        bad = [x["error"] for x in res if x["status"] != 0]
+        if bad:
+            raise RuntimeError(f"served fit failed: {bad[0]}")
+        lat = np.array([x["total_s"] for x in res])
+        grads = sum(x["grad_evals"] for x in res)
+        out.append({"arrival_rate_per_s": rate, "requests": n_req, "p50_s": round(float(np.percentile(lat, 50)), 4),
+                    "p99_s": round(float(np.percentile(lat, 99)), 4), "fits_per_s": round(n_req / wall, 3),
+                    "post_burnin_grad_evals_per_s": round(grads / wall, 1)})
+        log(f"[bench] serving at {rate}/s: p50 {out[-1]['p50_s']} s, p99 {out[-1]['p99_s']} s, "
+            f"{out[-1]['fits_per_s']} fits/s")
+    return {"workers": workers, "budget_gb": 200, "workload": "48^3 / 64x80x64 @1.5mm, 50% dual, fp32, "
+            "40+40 iterations, L=25", "rates": out}
+
+
 def cpu_baseline(p, workers):
     """Complex-state oracle (reference structure: c2c FFTs, separate W and W^T)."""
     import scipy.fft as sfft
@@ -179,6 +254,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-serve", action="store_true", help="skip the fit-server latency leg")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the --precision complex-state chain (for profiler runs)")
     a = ap.parse_args()
@@ -218,6 +296,9 @@ def main():
     fft_ms = sum(ks[k] for k in fft_names)
     wst = main_chain.w_stats()
     main_chain.close()
+    if a.calib_copy:
+        calibration_copy()
+    serving = None if a.main_only or a.no_serve else serving_leg()
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(p, workers=min(16, os.cpu_count() or 1))
@@ -249,8 +330,9 @@ def main():
         "kernel_ms_per_leapfrog_step": {k: round(v, 5) for k, v in ks.items()},
         "fft_passes_hbm_gbs": round(fft_bytes / (fft_ms * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, a.precision),
                      "algorithmic_bytes_per_launch": int(byt), "launch_ms": round(ks[dom], 5)},
+        "serving_latency": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
