@@ -225,6 +225,42 @@ def serving_leg(rates=(4.0, 16.0), n_req=24, workers=4, seed=20261202):
             "40+40 iterations, L=25", "rates": out}
 
 
+def batch_leg(n_chains=32, workers=8, seed=7):
+    """Many uneven dual-resolution chains at once through the fit server:
+    highres dims U{32..112} per axis at 1.5 mm (stdres 3 mm), theta =
+    (1, U(0.08, 0.2), U(0.8, 1.8)), 6.9 mm neighbourhood, smallest even
+    2,3,5,7-smooth embedding >= 2 max ijk per axis, 30 + 30 iterations of
+    L = 25 in FP32, all submitted together."""
+    from dualres_amd import problem, serve
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_chains):
+        dims = tuple(2 * int(rng.integers(16, 57)) for _ in range(3))
+        th = (1.0, float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 1.8)))
+        q = problem.dual_problem(dims, 1.5, th, seed_gp=500 + i)
+        mx = q["ijk"].max(0)
+        emb = tuple(problem.smooth_even(2 * int(m)) for m in mx)
+        reqs.append(serve.make_request(q["hr"], q["q_hr"], th, std=q["std"], q_std=q["q_std"], neighborhood=6.9,
+                                       burnin=30, nsave=30, thin=1, L=L_STEPS, seed=900 + i, precision="fp32",
+                                       embed=emb))
+    with serve.Server(200 * 10 ** 9, workers=workers) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(r) for r in reqs]
+        res = [srv.wait(i, timeout=900) for i in ids]
+        wall = time.perf_counter() - t0
+    bad = [x["error"] for x in res if x["status"] != 0]
+    if bad:
+        raise RuntimeError(f"batched fit failed: {bad[0]}")
+    grads = sum(x["grad_evals"] for x in res)
+    out = {"chains": n_chains, "workers": workers, "wall_s": round(wall, 2),
+           "post_burnin_grad_evals_per_s": round(grads / wall, 1), "fits_per_s": round(n_chains / wall, 3),
+           "workload": "dual-res, highres U{32..112}^3 @1.5mm, 6.9mm neighbourhood, smooth embeddings, fp32, "
+                       "30+30 iterations, L=25"}
+    log(f"[bench] batch of {n_chains} chains: {out['post_burnin_grad_evals_per_s']} post-burn-in grad evals/s, "
+        f"{wall:.1f} s")
+    return out
+
+
 def cpu_baseline(p, workers):
     """Complex-state oracle (reference structure: c2c FFTs, separate W and W^T)."""
     import scipy.fft as sfft
@@ -299,6 +335,7 @@ def main():
     if a.calib_copy:
         calibration_copy()
     serving = None if a.main_only or a.no_serve else serving_leg()
+    batch = None if a.main_only or a.no_serve else batch_leg()
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(p, workers=min(16, os.cpu_count() or 1))
@@ -333,6 +370,7 @@ def main():
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, a.precision),
                      "algorithmic_bytes_per_launch": int(byt), "launch_ms": round(ks[dom], 5)},
         "serving_latency": serving,
+        "batch_throughput": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

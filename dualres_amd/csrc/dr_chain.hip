@@ -232,8 +232,7 @@ struct Chain : public ChainBase {
     const AxisFFT<U> ad = axis_dev<U>(a);
     DR_REQUIRE(k_in >= 1 && k_in <= ad.M && k_out >= 1 && k_out <= ad.M && NB >= 1 && n_outer >= 1,
                DR_ELOGIC, "strided pass: bad extents");
-    const int tile = kTileStrided;
-    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, tile);
+    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, kTileStrided);
     const int TB = 1 << tbs;
     dim3 grid(cdiv(NB, TB), n_outer);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<U>);

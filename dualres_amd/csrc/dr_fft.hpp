@@ -37,6 +37,17 @@ constexpr int kEptX = 8;
 constexpr int kTileStrided = kEptStrided * kThreads;
 constexpr int kTileX = kEptX * kThreads;
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
+// by linear id.  xcd_tile maps the linear id to a tile index so that each
+// XCD receives a contiguous range of tiles: neighbouring tiles, which share
+// cache lines at their edges, then sit in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int lin, int total) {
+  const int x = lin % kXcds, j = lin / kXcds;
+  const int q = total / kXcds, r = total % kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 template <typename T>
 struct AxisFFT {
   int M;                  // transform length
@@ -227,8 +238,9 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];
   const int TB = 1 << tbs;
-  const int o = blockIdx.y;
-  const int b0 = blockIdx.x * TB;
+  const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int o = tile / gridDim.x;
+  const int b0 = (tile % gridDim.x) * TB;
   const int M = ax.M;
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
