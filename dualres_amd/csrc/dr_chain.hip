@@ -42,6 +42,22 @@ std::vector<int> factorize(int M) {
   return f;
 }
 
+// Transform lengths whose per-step passes use the compile-time stage
+// pipeline (fft_ct); other lengths take the runtime-radix path.
+template <class F>
+void with_fft_len(int M, F&& f) {
+  switch (M) {
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    case 160: f(std::integral_constant<int, 160>{}); break;
+    case 256: f(std::integral_constant<int, 256>{}); break;
+    case 320: f(std::integral_constant<int, 320>{}); break;
+    case 512: f(std::integral_constant<int, 512>{}); break;
+    default: f(std::integral_constant<int, 0>{}); break;
+  }
+}
+
 int pick_tbs(int len, int rows_lds, int elem_bytes, bool padded, int max_lines, int tile_elems) {
   int tbs = 0;
   auto lds = [&](int tb) { return (long)rows_lds * (padded ? tb + 1 : tb) * elem_bytes; };
@@ -237,8 +253,13 @@ struct Chain : public ChainBase {
     dim3 grid(cdiv(NB, TB), n_outer);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<U>);
     if (slot >= 0) note_blocks(slot, grid.x * grid.y);
-    hipLaunchKernelGGL((k_strided<U, INV, Pro, Epi>), grid, dim3(kThreads), lds, stream, ad, NB, k_in, k_out, tbs,
-                       pro, epi);
+    auto go = [&](auto mc) {
+      constexpr int MC = decltype(mc)::value;
+      hipLaunchKernelGGL((k_strided<U, INV, Pro, Epi, kEptStrided, MC>), grid, dim3(kThreads), lds, stream, ad, NB,
+                         k_in, k_out, tbs, pro, epi);
+    };
+    if constexpr (std::is_same<Pro, LoadS<U>>::value && std::is_same<Epi, StoreS<U>>::value) with_fft_len(ad.M, go);
+    else go(std::integral_constant<int, 0>{});
     DR_HIP(hipGetLastError());
     mark();
   }
@@ -251,7 +272,13 @@ struct Chain : public ChainBase {
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
     if (slot >= 0) note_blocks(slot, grid);
-    hipLaunchKernelGGL((k_r2c_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
+    auto go = [&](auto mc) {
+      constexpr int MC = decltype(mc)::value;
+      hipLaunchKernelGGL((k_r2c_x<U, Pro, Epi, kEptX, MC>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs,
+                         pro, epi);
+    };
+    if constexpr (std::is_same<Pro, LoadResidual<U>>::value) with_fft_len(Mh, go);
+    else go(std::integral_constant<int, 0>{});
     DR_HIP(hipGetLastError());
     mark();
   }
@@ -264,7 +291,13 @@ struct Chain : public ChainBase {
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
     if (slot >= 0) note_blocks(slot, grid);
-    hipLaunchKernelGGL((k_c2r_x<U, Pro, Epi>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs, pro, epi);
+    auto go = [&](auto mc) {
+      constexpr int MC = decltype(mc)::value;
+      hipLaunchKernelGGL((k_c2r_x<U, Pro, Epi, kEptX, MC>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs,
+                         pro, epi);
+    };
+    if constexpr (std::is_same<Pro, LoadRowC<U>>::value && std::is_same<Epi, StoreRowR<U>>::value) with_fft_len(Mh, go);
+    else go(std::integral_constant<int, 0>{});
     DR_HIP(hipGetLastError());
     mark();
   }
@@ -1127,8 +1160,12 @@ struct Chain : public ChainBase {
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<T>);
     if (mid_slot >= 0) note_blocks(mid_slot, grid);
     const long sk = (long)M[1] * H0;
-    hipLaunchKernelGGL((k_strided2<T, LoadS<T>, MidLeapfrog<T>, StoreS<T>>), dim3(grid), dim3(kThreads), lds, stream,
-                       ad, NB, reg[2], reg[2], tbs, LoadS<T>{{}, T1.p, 0, sk}, mid, StoreS<T>{{}, T1.p, 0, sk});
+    with_fft_len(ad.M, [&](auto mc) {
+      constexpr int MC = decltype(mc)::value;
+      hipLaunchKernelGGL((k_strided2<T, LoadS<T>, MidLeapfrog<T>, StoreS<T>, kEptStrided, MC>), dim3(grid),
+                         dim3(kThreads), lds, stream, ad, NB, reg[2], reg[2], tbs, LoadS<T>{{}, T1.p, 0, sk}, mid,
+                         StoreS<T>{{}, T1.p, 0, sk});
+    });
     DR_HIP(hipGetLastError());
     mark();
   }

@@ -206,6 +206,81 @@ __device__ __forceinline__ void fft_stage(cx<T>* lds, int P, int tbs, const Axis
   __syncthreads();
 }
 
+// The same stages with the length, span and twiddle stride known at compile
+// time (the hot passes at common lengths): all index arithmetic folds.
+struct Factors {
+  int n;
+  int r[kMaxStages];
+};
+constexpr Factors factor_ct(int M) {  // the radix order of factorize()
+  Factors f{};
+  int m = M;
+  while (m % 8 == 0 && m > 8) { f.r[f.n++] = 8; m /= 8; }
+  if (m % 8 == 0) { f.r[f.n++] = 8; m /= 8; }
+  while (m % 4 == 0) { f.r[f.n++] = 4; m /= 4; }
+  while (m % 2 == 0) { f.r[f.n++] = 2; m /= 2; }
+  while (m % 3 == 0) { f.r[f.n++] = 3; m /= 3; }
+  while (m % 5 == 0) { f.r[f.n++] = 5; m /= 5; }
+  while (m % 7 == 0) { f.r[f.n++] = 7; m /= 7; }
+  return f;
+}
+
+template <typename T, int R, bool INV, int EPT, int M, int NS, int TWS>
+__device__ __forceinline__ void fft_stage_ct(cx<T>* lds, int P, int tbs, const cx<T>* __restrict__ tw) {
+  const int TB = 1 << tbs;
+  constexpr int BF = M / R;
+  const int total = BF << tbs;
+  constexpr int estep = (M / (NS * R)) * TWS;
+  constexpr int IT = (EPT + R - 1) / R;
+  cx<T> v[IT][R];
+  int dst[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int q = threadIdx.x + it * kThreads;
+    dst[it] = -1;
+    if (q < total) {
+      const int b = q & (TB - 1);
+      const int j = q >> tbs;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = lds[(j + r * BF) * P + b];
+      const int jm = j % NS;
+      if (NS > 1 && jm != 0) {
+        const int e = jm * estep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cx<T> w = tw[r * e];
+          v[it][r] = INV ? cmulc(v[it][r], w) : cmul(v[it][r], w);
+        }
+      }
+      Dft<T, R, INV>::run(v[it]);
+      dst[it] = ((j - jm) * R + jm) * P + b;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (dst[it] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dst[it] + r * NS * P] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, bool INV, int EPT, int M, int TWS, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_ct(cx<T>* lds, int P, int tbs, const cx<T>* tw) {
+  constexpr Factors F = factor_ct(M);
+  if constexpr (S < F.n) {
+    constexpr int R = F.r[S];
+    fft_stage_ct<T, R, INV, EPT, M, NS, TWS>(lds, P, tbs, tw);
+    fft_ct<T, INV, EPT, M, TWS, S + 1, NS * R>(lds, P, tbs, tw);
+  }
+}
+
+// runtime-length stages, or the compile-time pipeline when MC > 0 (MC == ax.M)
+template <typename T, bool INV, int EPT, int MC, int TWS>
+__device__ __forceinline__ void fft_run(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax);
+
 template <typename T, bool INV, int EPT>
 __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax) {
   int Ns = 1;
@@ -224,6 +299,12 @@ __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFF
   }
 }
 
+template <typename T, bool INV, int EPT, int MC, int TWS>
+__device__ __forceinline__ void fft_run(cx<T>* lds, int P, int tbs, const AxisFFT<T>& ax) {
+  if constexpr (MC > 0) fft_ct<T, INV, EPT, MC, TWS>(lds, P, tbs, ax.tw);
+  else fft_lds<T, INV, EPT>(lds, P, tbs, ax);
+}
+
 // ---------------------------------------------------------------- pass kernels
 // Functor interfaces:
 //   strided:  cx<T> Pro::load(int o, int k, int b);   void Epi::store(int o, int k, int b, cx<T>);
@@ -231,7 +312,7 @@ __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFF
 //   c2r_x:    cx<T> Pro::load(int row, int k);   void Epi::store2(int row, int n, cx<T> z)  (samples 2n, 2n+1)
 // and void finish(double* red, int block) on both for optional reductions.
 
-template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided>
+template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -241,7 +322,7 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
   const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const int o = tile / gridDim.x;
   const int b0 = (tile % gridDim.x) * TB;
-  const int M = ax.M;
+  const int M = MC > 0 ? MC : ax.M;
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     cx<T> v = mk<T>(0, 0);
@@ -249,7 +330,7 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
     lds[k * TB + b] = v;
   }
   __syncthreads();
-  fft_lds<T, INV, EPT>(lds, TB, tbs, ax);
+  fft_run<T, INV, EPT, MC, 1>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b]);
@@ -263,7 +344,7 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
 // the leapfrog kick and drift in the spectral domain sit between the forward
 // z pass of one gradient and the inverse z pass of the next.
 //   cx<T> Mid::apply(int k, int b, cx<T> v)  for every k < M.
-template <typename T, class Pro, class Mid, class Epi, int EPT = kEptStrided>
+template <typename T, class Pro, class Mid, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -271,7 +352,7 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
   __shared__ double red[kThreads];
   const int TB = 1 << tbs;
   const int b0 = blockIdx.x * TB;
-  const int M = ax.M;
+  const int M = MC > 0 ? MC : ax.M;
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     cx<T> v = mk<T>(0, 0);
@@ -279,13 +360,13 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
     lds[k * TB + b] = v;
   }
   __syncthreads();
-  fft_lds<T, false, EPT>(lds, TB, tbs, ax);
+  fft_run<T, false, EPT, MC, 1>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) lds[k * TB + b] = mid.apply(k, b0 + b, lds[k * TB + b]);
   }
   __syncthreads();
-  fft_lds<T, true, EPT>(lds, TB, tbs, ax);
+  fft_run<T, true, EPT, MC, 1>(lds, TB, tbs, ax);
   for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
     if (b0 + b < NB) epi.store(0, k, b0 + b, lds[k * TB + b]);
@@ -299,7 +380,7 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
 // Real rows of length M0 (first n_in samples nonzero) -> M0/2+1 complex.
 // ax describes the complex FFT of length Mh = M0/2 with tws = 2 into the
 // length-M0 table, so ax.tw[k] = exp(-2 pi i k / M0).
-template <typename T, class Pro, class Epi, int EPT = kEptX>
+template <typename T, class Pro, class Epi, int EPT = kEptX, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -307,7 +388,7 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   __shared__ double red[kThreads];
   const int TB = 1 << tbs;
   const int P = TB + 1;
-  const int Mh = ax.M;
+  const int Mh = MC > 0 ? MC : ax.M;
   const int row0 = blockIdx.x * TB;
   for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
     const int l = idx / Mh, n = idx - l * Mh;
@@ -317,7 +398,7 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
     lds[n * P + l] = v;
   }
   __syncthreads();
-  fft_lds<T, false, EPT>(lds, P, tbs, ax);
+  fft_run<T, false, EPT, MC, 2>(lds, P, tbs, ax);
   const int H = Mh + 1;
   for (int idx = threadIdx.x; idx < TB * H; idx += kThreads) {
     const int l = idx / H, k = idx - l * H;
@@ -339,7 +420,7 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
 
 // M0/2+1 complex per row -> real rows; the epilogue receives packed pairs
 // (x[2n], x[2n+1]) = (Re z[n], Im z[n]) unscaled.
-template <typename T, class Pro, class Epi, int EPT = kEptX>
+template <typename T, class Pro, class Epi, int EPT = kEptX, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   extern __shared__ unsigned char smem_raw[];
@@ -347,7 +428,7 @@ k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   __shared__ double red[kThreads];
   const int TB = 1 << tbs;
   const int P = TB + 1;
-  const int Mh = ax.M;
+  const int Mh = MC > 0 ? MC : ax.M;
   const int H = Mh + 1;
   const int row0 = blockIdx.x * TB;
   for (int idx = threadIdx.x; idx < TB * H; idx += kThreads) {
@@ -375,7 +456,7 @@ k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
     lds[k * P + l] = zk;
   }
   __syncthreads();
-  fft_lds<T, true, EPT>(lds, P, tbs, ax);
+  fft_run<T, true, EPT, MC, 2>(lds, P, tbs, ax);
   for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
     const int l = idx / Mh, n = idx - l * Mh;
     const int row = row0 + l;
