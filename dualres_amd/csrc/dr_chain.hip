@@ -258,8 +258,7 @@ struct Chain : public ChainBase {
       hipLaunchKernelGGL((k_strided<U, INV, Pro, Epi, kEptStrided, MC>), grid, dim3(kThreads), lds, stream, ad, NB,
                          k_in, k_out, tbs, pro, epi);
     };
-    if constexpr (std::is_same<Pro, LoadS<U>>::value && std::is_same<Epi, StoreS<U>>::value) with_fft_len(ad.M, go);
-    else go(std::integral_constant<int, 0>{});
+    with_fft_len(ad.M, go);
     DR_HIP(hipGetLastError());
     mark();
   }
@@ -277,8 +276,7 @@ struct Chain : public ChainBase {
       hipLaunchKernelGGL((k_r2c_x<U, Pro, Epi, kEptX, MC>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs,
                          pro, epi);
     };
-    if constexpr (std::is_same<Pro, LoadResidual<U>>::value) with_fft_len(Mh, go);
-    else go(std::integral_constant<int, 0>{});
+    with_fft_len(Mh, go);
     DR_HIP(hipGetLastError());
     mark();
   }
@@ -296,8 +294,7 @@ struct Chain : public ChainBase {
       hipLaunchKernelGGL((k_c2r_x<U, Pro, Epi, kEptX, MC>), dim3(grid), dim3(kThreads), lds, stream, ad, nrows, tbs,
                          pro, epi);
     };
-    if constexpr (std::is_same<Pro, LoadRowC<U>>::value && std::is_same<Epi, StoreRowR<U>>::value) with_fft_len(Mh, go);
-    else go(std::integral_constant<int, 0>{});
+    with_fft_len(Mh, go);
     DR_HIP(hipGetLastError());
     mark();
   }
