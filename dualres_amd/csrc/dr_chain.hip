@@ -192,7 +192,7 @@ struct Chain : public ChainBase {
   DevBuf<cx<T>> muA, muB, p, miA, miB, pim, T1, T2;
   DevBuf<T> invlam;
   DevBuf<double> lam;  // Re lambda on the half spectrum (float64)
-  DevBuf<T> RA, RB, Yg, WTv, Rf, zbuf;
+  DevBuf<T> RA, RB, Yg, WTv, WTvb, Rf, zbuf;
   DevBuf<uint8_t> mask;
   DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
   DevBuf<T> Wvals, WTvals, Ysd, vsd, outbuf;
@@ -1022,6 +1022,8 @@ struct Chain : public ChainBase {
       vsd.alloc(n_s);
       WTv.alloc(nreg);
       WTv.zero(stream);
+      WTvb.alloc(nreg);
+      WTvb.zero(stream);
       std::vector<T> ys(n_s);
       for (long i = 0; i < n_s; ++i) ys[i] = (T)Ys[i];
       DR_HIP(hipMemcpyAsync(Ysd.p, ys.data(), n_s * sizeof(T), hipMemcpyHostToDevice, stream));
@@ -1093,31 +1095,21 @@ struct Chain : public ChainBase {
     if (use_cls) {
       const int gi = cdiv((long)ws.n_int * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64);
       if (slot >= 0) note_blocks(slot, gi + gb);
-      double* pp = slot >= 0 ? pslot(slot) : nullptr;
       const int R0 = reg[0], R01 = reg[0] * reg[1];
-      if (ws.n_int > 0) {
-        hipLaunchKernelGGL(k_wmu_int<T>, dim3(gi), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream,
-                           ws.n_int, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, R, Ysd.p,
-                           (T)sigma[1], ws.Vl.p, pp);
+      if (gi + gb > 0) {
+        const WmuInt<T> ai{ws.n_int, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, ws.Vl.p};
+        const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01,
+                           ws.Vb.p};
+        hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream, ai,
+                           ab, gi, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
         DR_HIP(hipGetLastError());
       }
       mark();
-      if (ws.n_bnd > 0) {
-        hipLaunchKernelGGL(k_wmu_bnd<T>, dim3(gb), dim3(kThreads), 0, stream, ws.n_bnd, ws.brow.p, ws.brp.p,
-                           ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01, R, Ysd.p, (T)sigma[1], ws.Vb.p,
-                           pp ? pp + gi : nullptr);
-        DR_HIP(hipGetLastError());
-      }
-      mark();
-      hipLaunchKernelGGL(k_wtv_int<T>, dim3((unsigned)(ws.nvpad / kThreads)), dim3(kThreads), 0, stream, ws.vg.p,
-                         ws.vub.p, ws.wcls.p, ws.sub_start.p, ws.sub_dl.p, ws.sub_w.p, ws.Vl.p, WTv.p);
+      const int ti = (int)(ws.nvpad / kThreads), tb = cdiv(ws.n_touch, kThreads / 16);
+      const WtvInt<T> wi{ws.vg.p, ws.vub.p, ws.wcls.p, ws.sub_start.p, ws.sub_dl.p, ws.sub_w.p, ws.Vl.p, WTv.p};
+      const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p};
+      hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), 0, stream, wi, wb, ti);
       DR_HIP(hipGetLastError());
-      mark();
-      if (ws.n_touch > 0) {
-        hipLaunchKernelGGL(k_wtv_bnd<T>, dim3(cdiv(ws.n_touch, kThreads / 16)), dim3(kThreads), 0, stream,
-                           ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTv.p);
-        DR_HIP(hipGetLastError());
-      }
       mark();
       return;
     }
@@ -1132,7 +1124,8 @@ struct Chain : public ChainBase {
     mark();
   }
   LoadResidual<T> residual(const T* R, int slot) {
-    return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr, (T)sigma[0], reg[0],
+    return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr,
+                           cfg.n_datasets == 2 && use_cls ? WTvb.p : nullptr, (T)sigma[0], reg[0],
                            slot >= 0 ? pslot(slot) : nullptr, 0.0};
   }
   // inverse y and x passes from T1 (z < n2) to the real region field R

@@ -255,13 +255,14 @@ template <typename T> struct LoadNoiseBuf {
 // (MultiResParameters.h:357-374): r = sigma_h^-2 (Yh - mu) + W^T v at data
 // voxels, 0 elsewhere; accumulates ||Yh - mu[I]||^2 (:502-503).
 template <typename T> struct LoadResidual {
-  const T* R; const T* Yg; const uint8_t* mask; const T* WTv; T sig; int n0; double* part; double acc;
+  const T* R; const T* Yg; const uint8_t* mask; const T* WTv; const T* WTvb; T sig; int n0; double* part; double acc;
   __device__ __forceinline__ T one(long g) {
     if (!mask[g]) return (T)0;
     const T d = Yg[g] - R[g];
     acc += (double)d * (double)d;
     T r = sig * d;
     if (WTv) r += WTv[g];
+    if (WTvb) r += WTvb[g];
     return r;
   }
   __device__ __forceinline__ cx<T> load2(int row, int n) {
@@ -377,86 +378,109 @@ struct alignas(4 * sizeof(T)) Quad {
 __device__ __forceinline__ unsigned lo16(unsigned x) { return x & 0xffffu; }
 __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 
+// Argument bundles of the four parts of the two products.
+template <typename T> struct WmuInt {
+  int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
+};
+template <typename T> struct WmuBnd {
+  int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
+  int R0, R01; T* Vb;
+};
+template <typename T> struct WtvInt {
+  const int* g; const int* ub; const int* wcls; const int* sub_start; const int* sub_dl; const T* sub_w;
+  const T* Vl; T* out;
+};
+template <typename T> struct WtvBnd {
+  int n; const int* g; const int* rp; const uint16_t* d; const T* w; int R0, R01, dc; const T* Vb; T* out;
+};
+
 // interior rows: 8 lanes per row, lane q takes stencil entries q, q+8, ...
 // (consecutive lanes read consecutive entries of an x-run) from the stencil
 // table staged in LDS (nD * (4 + sizeof(T)) bytes), issuing batches of 8
 // independent gathers before their multiply-adds; the 8 partial sums are
-// combined by a fixed butterfly
+// combined by a fixed butterfly.  Returns this lane's share of ||Ys - W R||^2.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-k_wmu_int(int n, const int* __restrict__ row, const int* __restrict__ base, const int* __restrict__ lidx, int nD,
-          const int* __restrict__ doff, const T* __restrict__ w, const T* __restrict__ R, const T* __restrict__ Ys,
-          T sig_s, T* __restrict__ Vl, double* part) {
-  __shared__ double red[kThreads];
+__device__ __forceinline__ double wmu_int_block(const WmuInt<T>& a, int blk, const T* __restrict__ R,
+                                                const T* __restrict__ Ys, T sig_s) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* sw = reinterpret_cast<T*>(lds_raw);
-  int* so = reinterpret_cast<int*>(lds_raw + (size_t)nD * sizeof(T));
-  for (int s = threadIdx.x; s < nD; s += kThreads) {
-    sw[s] = w[s];
-    so[s] = doff[s];
+  int* so = reinterpret_cast<int*>(lds_raw + (size_t)a.nD * sizeof(T));
+  for (int s = threadIdx.x; s < a.nD; s += kThreads) {
+    sw[s] = a.w[s];
+    so[s] = a.doff[s];
   }
   __syncthreads();
   const int q = threadIdx.x & 7;
-  const int k = (blockIdx.x * kThreads + threadIdx.x) >> 3;
+  const int k = (blk * kThreads + (int)threadIdx.x) >> 3;
   T acc = (T)0;
-  if (k < n) {
-    const T* Rb = R + base[k];
+  if (k < a.n) {
+    const T* Rb = R + a.base[k];
     constexpr int B = 8;
     int s = q;
-    for (; s + 8 * (B - 1) < nD; s += 8 * B) {
+    for (; s + 8 * (B - 1) < a.nD; s += 8 * B) {
       T v[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) v[b] = Rb[so[s + 8 * b]];
 #pragma unroll
       for (int b = 0; b < B; ++b) acc += sw[s + 8 * b] * v[b];
     }
-    for (; s < nD; s += 8) acc += sw[s] * Rb[so[s]];
+    for (; s < a.nD; s += 8) acc += sw[s] * Rb[so[s]];
   }
   acc += __shfl_xor(acc, 1, 8);
   acc += __shfl_xor(acc, 2, 8);
   acc += __shfl_xor(acc, 4, 8);
   double e2 = 0.0;
-  if (k < n && q == 0) {
-    const T e = Ys[row[k]] - acc;
-    Vl[lidx[k]] = sig_s * e;
+  if (k < a.n && q == 0) {
+    const T e = Ys[a.row[k]] - acc;
+    a.Vl[a.lidx[k]] = sig_s * e;
     e2 = (double)e * (double)e;
   }
-  if (part) red_write(e2, red, part, blockIdx.x);
+  return e2;
 }
 
 // boundary rows, one wavefront per row; entry lists start on multiples of 4
 // and are padded to whole quads (offset 0, weight 0)
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-k_wmu_bnd(int n, const int* __restrict__ row, const int* __restrict__ rp, const int* __restrict__ base,
-          const uint16_t* __restrict__ d, const T* __restrict__ w, const int* __restrict__ anchor, int R0, int R01,
-          const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, T* __restrict__ Vb, double* part) {
-  __shared__ double red[kThreads];
+__device__ __forceinline__ double wmu_bnd_block(const WmuBnd<T>& a, int blk, const T* __restrict__ R,
+                                                const T* __restrict__ Ys, T sig_s) {
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int k = blk * (kThreads / 64) + (threadIdx.x >> 6);
   double e2 = 0.0;
-  if (k < n) {
-    const int j1 = rp[k + 1] >> 2;
-    const T* Rb = R + base[k];
-    const uint2* d4 = reinterpret_cast<const uint2*>(d);
-    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(w);
+  if (k < a.n) {
+    const int j1 = a.rp[k + 1] >> 2;
+    const T* Rb = R + a.base[k];
+    const uint2* d4 = reinterpret_cast<const uint2*>(a.d);
+    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(a.w);
     T acc = (T)0;
-    for (int j = (rp[k] >> 2) + lane; j < j1; j += 64) {
+    for (int j = (a.rp[k] >> 2) + lane; j < j1; j += 64) {
       const uint2 dq = d4[j];
       const Quad<T> wq = w4[j];
-      acc += wq.v[0] * Rb[unpack3(lo16(dq.x), R0, R01)];
-      acc += wq.v[1] * Rb[unpack3(hi16(dq.x), R0, R01)];
-      acc += wq.v[2] * Rb[unpack3(lo16(dq.y), R0, R01)];
-      acc += wq.v[3] * Rb[unpack3(hi16(dq.y), R0, R01)];
+      acc += wq.v[0] * Rb[unpack3(lo16(dq.x), a.R0, a.R01)];
+      acc += wq.v[1] * Rb[unpack3(hi16(dq.x), a.R0, a.R01)];
+      acc += wq.v[2] * Rb[unpack3(lo16(dq.y), a.R0, a.R01)];
+      acc += wq.v[3] * Rb[unpack3(hi16(dq.y), a.R0, a.R01)];
     }
     acc = wave_sum(acc);
     if (lane == 0) {
-      const T e = Ys[row[k]] - acc;
-      const int a = anchor[k];
-      if (a >= 0) Vb[a] = sig_s * e;
+      const T e = Ys[a.row[k]] - acc;
+      const int an = a.anchor[k];
+      if (an >= 0) a.Vb[an] = sig_s * e;
       e2 = (double)e * (double)e;
     }
   }
+  return e2;
+}
+
+// v = sigma_s (Ys - W R) for every row and ||Ys - W R||^2: blocks [0, gi)
+// take the interior rows, the rest the boundary rows, in one launch so the
+// latency-bound gathers and the streaming boundary entries overlap
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, int gi, const T* __restrict__ R, const T* __restrict__ Ys, T sig_s,
+         double* part) {
+  __shared__ double red[kThreads];
+  const int blk = blockIdx.x;
+  const double e2 = blk < gi ? wmu_int_block(ai, blk, R, Ys, sig_s) : wmu_bnd_block(ab, blk - gi, R, Ys, sig_s);
   if (part) red_write(e2, red, part, blockIdx.x);
 }
 
@@ -466,58 +490,61 @@ k_wmu_bnd(int n, const int* __restrict__ row, const int* __restrict__ rp, const 
 // batches of 16 independent gathers; padding lanes (g < 0) compute on a
 // valid lattice base and store nothing.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-k_wtv_int(const int* __restrict__ g, const int* __restrict__ ub, const int* __restrict__ wcls,
-          const int* __restrict__ sub_start, const int* __restrict__ sub_dl, const T* __restrict__ sub_w,
-          const T* __restrict__ Vl, T* __restrict__ out) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
-  const int cls = __builtin_amdgcn_readfirstlane(wcls[t >> 6]);
-  const int s0 = sub_start[cls], s1 = sub_start[cls + 1];
-  const T* V = Vl + ub[t];
+__device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
+  const int t = blk * kThreads + threadIdx.x;
+  const int cls = __builtin_amdgcn_readfirstlane(a.wcls[t >> 6]);
+  const int s0 = a.sub_start[cls], s1 = a.sub_start[cls + 1];
+  const T* V = a.Vl + a.ub[t];
   T acc = (T)0;
   constexpr int B = 16;
   int s = s0;
   for (; s + B <= s1; s += B) {
     T v[B];
 #pragma unroll
-    for (int b = 0; b < B; ++b) v[b] = V[sub_dl[s + b]];
+    for (int b = 0; b < B; ++b) v[b] = V[a.sub_dl[s + b]];
 #pragma unroll
-    for (int b = 0; b < B; ++b) acc += sub_w[s + b] * v[b];
+    for (int b = 0; b < B; ++b) acc += a.sub_w[s + b] * v[b];
   }
-  for (; s < s1; ++s) acc += sub_w[s] * V[sub_dl[s]];
-  const int gj = g[t];
-  if (gj >= 0) out[gj] = acc;
+  for (; s < s1; ++s) acc += a.sub_w[s] * V[a.sub_dl[s]];
+  const int gj = a.g[t];
+  if (gj >= 0) a.out[gj] = acc;
 }
 
-// W^T v of the boundary rows, added at the voxels they touch; 16 lanes per
-// voxel, quads of entries as in k_wmu_bnd (pad offset = the voxel itself,
-// weight 0).  Entry offsets are voxel - anchor + 15 per axis, packed.
+// W^T v of the boundary rows at the voxels they touch (written to its own
+// field, which the residual adds); 16 lanes per voxel, quads of entries as in
+// the W mu part (pad offset = the voxel itself, weight 0).  Entry offsets are
+// voxel - anchor + 15 per axis, packed.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-k_wtv_bnd(int n, const int* __restrict__ g, const int* __restrict__ rp, const uint16_t* __restrict__ d,
-          const T* __restrict__ w, int R0, int R01, int dc, const T* __restrict__ Vb, T* __restrict__ out) {
+__device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
   const int lane = threadIdx.x & 15;
-  const int k = blockIdx.x * (kThreads / 16) + (threadIdx.x >> 4);
+  const int k = blk * (kThreads / 16) + (threadIdx.x >> 4);
   T acc = (T)0;
   int gj = 0;
-  if (k < n) {
-    gj = g[k];
-    const int j1 = rp[k + 1] >> 2;
-    const T* Vg = Vb + (gj + dc);
-    const uint2* d4 = reinterpret_cast<const uint2*>(d);
-    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(w);
-    for (int j = (rp[k] >> 2) + lane; j < j1; j += 16) {
+  if (k < a.n) {
+    gj = a.g[k];
+    const int j1 = a.rp[k + 1] >> 2;
+    const T* Vg = a.Vb + (gj + a.dc);
+    const uint2* d4 = reinterpret_cast<const uint2*>(a.d);
+    const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(a.w);
+    for (int j = (a.rp[k] >> 2) + lane; j < j1; j += 16) {
       const uint2 dq = d4[j];
       const Quad<T> wq = w4[j];
-      acc += wq.v[0] * Vg[-unpack3(lo16(dq.x), R0, R01)];
-      acc += wq.v[1] * Vg[-unpack3(hi16(dq.x), R0, R01)];
-      acc += wq.v[2] * Vg[-unpack3(lo16(dq.y), R0, R01)];
-      acc += wq.v[3] * Vg[-unpack3(hi16(dq.y), R0, R01)];
+      acc += wq.v[0] * Vg[-unpack3(lo16(dq.x), a.R0, a.R01)];
+      acc += wq.v[1] * Vg[-unpack3(hi16(dq.x), a.R0, a.R01)];
+      acc += wq.v[2] * Vg[-unpack3(lo16(dq.y), a.R0, a.R01)];
+      acc += wq.v[3] * Vg[-unpack3(hi16(dq.y), a.R0, a.R01)];
     }
   }
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
-  if (k < n && lane == 0) out[gj] += acc;
+  if (k < a.n && lane == 0) a.out[gj] = acc;
+}
+
+// W^T v: blocks [0, gi) interior voxels, the rest boundary voxels
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_wtv_sb(WtvInt<T> ai, WtvBnd<T> ab, int gi) {
+  if ((int)blockIdx.x < gi) wtv_int_block(ai, blockIdx.x);
+  else wtv_bnd_block(ab, blockIdx.x - gi);
 }
 
 // mcmc_summary<T>::update (gaussian_process_model.h:205-224) at the output

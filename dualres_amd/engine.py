@@ -218,10 +218,7 @@ class Chain:
         L.check(self._lib.dr_profile_kernels(self._h, L.dptr(ms), 16, C.byref(nk), C.byref(ns)))
         names = ["z_fft_kick_drift", "c2r_y", "c2r_x"]
         if self.n_datasets == 2:
-            if self.w_stats()["stencil_layout"]:
-                names += ["W_mu_int", "W_mu_bnd", "Wt_v_int", "Wt_v_bnd"]
-            else:
-                names += ["W_mu", "Wt_v"]
+            names += ["W_mu", "Wt_v"]
         names += ["r2c_x_residual", "r2c_y"]
         return dict(zip(names, ms[: nk.value].tolist())), ns.value
 
