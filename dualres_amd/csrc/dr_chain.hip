@@ -897,7 +897,7 @@ struct Chain : public ChainBase {
         }
       }
     }
-    for (long d = 0; d < n_h; ++d) interleave_quads(tdall, twall, tstart[d], tstart[d + 1], 16);
+    for (long d = 0; d < n_h; ++d) interleave_quads(tdall, twall, tstart[d], tstart[d + 1], kWtvLanes);
     std::vector<int32_t> tg, trp{0};
     for (long d = 0; d < n_h; ++d)
       if (tstart[d + 1] > tstart[d]) {
@@ -1105,7 +1105,7 @@ struct Chain : public ChainBase {
         DR_HIP(hipGetLastError());
       }
       mark();
-      const int ti = (int)(ws.nvpad / kThreads), tb = cdiv(ws.n_touch, kThreads / 16);
+      const int ti = (int)(ws.nvpad / kThreads), tb = cdiv(ws.n_touch, kThreads / kWtvLanes);
       const WtvInt<T> wi{ws.vg.p, ws.vub.p, ws.wcls.p, ws.sub_start.p, ws.sub_dl.p, ws.sub_w.p, ws.Vl.p, WTv.p};
       const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p};
       hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), 0, stream, wi, wb, ti);

@@ -378,6 +378,10 @@ struct alignas(4 * sizeof(T)) Quad {
 __device__ __forceinline__ unsigned lo16(unsigned x) { return x & 0xffffu; }
 __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 
+// lanes per voxel of the boundary W^T v part (the transposed entry lists are
+// interleaved in blocks of this many quads)
+constexpr int kWtvLanes = 8;
+
 // Argument bundles of the four parts of the two products.
 template <typename T> struct WmuInt {
   int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
@@ -516,8 +520,8 @@ __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
 // voxel - anchor + 15 per axis, packed.
 template <typename T>
 __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
-  const int lane = threadIdx.x & 15;
-  const int k = blk * (kThreads / 16) + (threadIdx.x >> 4);
+  const int lane = threadIdx.x & (kWtvLanes - 1);
+  const int k = blk * (kThreads / kWtvLanes) + (threadIdx.x / kWtvLanes);
   T acc = (T)0;
   int gj = 0;
   if (k < a.n) {
@@ -526,7 +530,7 @@ __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
     const T* Vg = a.Vb + (gj + a.dc);
     const uint2* d4 = reinterpret_cast<const uint2*>(a.d);
     const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(a.w);
-    for (int j = (a.rp[k] >> 2) + lane; j < j1; j += 16) {
+    for (int j = (a.rp[k] >> 2) + lane; j < j1; j += kWtvLanes) {
       const uint2 dq = d4[j];
       const Quad<T> wq = w4[j];
       acc += wq.v[0] * Vg[-unpack3(lo16(dq.x), a.R0, a.R01)];
@@ -536,7 +540,7 @@ __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
     }
   }
 #pragma unroll
-  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+  for (int o = kWtvLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWtvLanes);
   if (k < a.n && lane == 0) a.out[gj] = acc;
 }
 
