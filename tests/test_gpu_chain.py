@@ -253,3 +253,35 @@ def test_device_summary_matches_host_moments(gpu_available, precision):
     c.set_output_indices(out)          # resets the running sums
     assert c.summary_read()[3] == 0 and not c.summary_read()[0].any()
     c.close()
+
+
+def test_fp32_chain_tracks_fp64(gpu_available):
+    """FP32 gradients and FFTs with FP64 energies and accept: driven by the
+    same injected normals, uniforms and Gibbs draws, the FP32 chain stays within
+    1e-3 (relative) of the FP64 chain over 40 dual-resolution HMC iterations,
+    posterior running mean included."""
+    from dualres_amd.sampler import McmcSummary
+    p = small_dual(dims=(24, 20, 16))
+    chains = {pr: engine_chain(p, THETA2, precision=pr, state_mode="compat", seed=3) for pr in ("fp64", "fp32")}
+    summ = {}
+    for c in chains.values():
+        c.add_data_to_mu()
+        c.set_output_indices(p["ijk"])
+        c.set_sigma_sq_inv([1.3, 0.7])
+    for pr, c in chains.items():
+        summ[pr] = McmcSummary(c.n_out, 2, chain=c)
+    rng = np.random.default_rng(21)
+    for it in range(40):
+        z = (rng.standard_normal(chains["fp64"].N), rng.standard_normal(chains["fp64"].N))
+        u = rng.uniform()
+        r = {pr: c.update(0.08, 6, u, noise=z) for pr, c in chains.items()}
+        assert r["fp32"]["accepted"] == r["fp64"]["accepted"], it
+        g = (rng.gamma(20.0, 0.06), rng.gamma(10.0, 0.07))
+        for pr, c in chains.items():
+            c.set_sigma_sq_inv(list(g))
+            summ[pr].update(None, np.ones(2), 0.0)
+    m64, m32 = chains["fp64"].mu(), chains["fp32"].mu()
+    assert rel(m32, m64) < 1e-3
+    assert rel(summ["fp32"].mode_mu(), summ["fp64"].mode_mu()) < 1e-3
+    for c in chains.values():
+        c.close()
