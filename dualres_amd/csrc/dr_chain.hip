@@ -203,7 +203,7 @@ struct Chain : public ChainBase {
   int64_t nsum = 0;
   // stencil + boundary W (see build_stencil)
   struct WStencil {
-    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_dl, vg, vub, wcls;
+    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_dl, vg, vub, wcls, run_start, run_dl0, run_len, run_wo;
     DevBuf<T> w, sub_w, Vl;
     DevBuf<int> brow, brp, bbase, banchor, tg, trp;
     DevBuf<uint16_t> bd, td;
@@ -784,7 +784,11 @@ struct Chain : public ChainBase {
       for (int a = 0; a < 3; ++a) u[a] = (b[a] - c[a]) / sg[a];
       ilidx[k] = (int32_t)flatL(u);
     }
-    // 3. data voxels bucketed by residue class, buckets padded to wavefronts
+    // 3. data voxels bucketed by residue class; within a bucket, up to 4
+    //    voxels at consecutive lattice indices form one lane's group (their
+    //    reads of a sub-stencil x-run overlap); groups padded to wavefronts.
+    //    The sub-stencil of each class is cut into x-runs of at most 8 entries
+    //    (consecutive entries whose lattice offset falls by one).
     std::vector<std::vector<int32_t>> bg(ncls), bu(ncls);
     long safe_ub = -1;
     std::vector<int32_t> gidx(n_h);
@@ -803,21 +807,43 @@ struct Chain : public ChainBase {
       bu[k].push_back((int32_t)f);
     }
     if (safe_ub < 0) safe_ub = 0;
-    std::vector<int32_t> vg, vub, wcls;
+    std::vector<int32_t> gub, gg, wcls;
+    auto push_group = [&](int32_t ub, const int32_t* g4) {
+      gub.push_back(ub);
+      gg.insert(gg.end(), g4, g4 + 4);
+    };
+    const int32_t none4[4] = {-1, -1, -1, -1};
     for (int k = 0; k < ncls; ++k) {
       if (bg[k].empty()) continue;
-      vg.insert(vg.end(), bg[k].begin(), bg[k].end());
-      vub.insert(vub.end(), bu[k].begin(), bu[k].end());
-      while (vg.size() % 64) {
-        vg.push_back(-1);
-        vub.push_back((int32_t)safe_ub);
+      const size_t n = bg[k].size(), g0 = gub.size();
+      for (size_t i = 0; i < n;) {
+        int32_t g4[4] = {bg[k][i], -1, -1, -1};
+        int cnt = 1;
+        while (cnt < 4 && i + cnt < n && bu[k][i + cnt] == bu[k][i] + cnt) {
+          g4[cnt] = bg[k][i + cnt];
+          ++cnt;
+        }
+        push_group(bu[k][i], g4);
+        i += cnt;
       }
-      for (size_t w = 0; w < bg[k].size(); w += 64) wcls.push_back(k);
+      while ((gub.size() - g0) % 64) push_group((int32_t)safe_ub, none4);
+      for (size_t w = g0; w < gub.size(); w += 64) wcls.push_back(k);
     }
-    while (vg.size() % kThreads || vg.empty()) {
-      vg.push_back(-1);
-      vub.push_back((int32_t)safe_ub);
-      if (vg.size() % 64 == 1) wcls.push_back(0);
+    while (gub.size() % kThreads || gub.empty()) {
+      push_group((int32_t)safe_ub, none4);
+      if (gub.size() % 64 == 1) wcls.push_back(0);
+    }
+    std::vector<int32_t> run_start(ncls + 1, 0), run_dl0, run_len, run_wo;
+    for (int k = 0; k < ncls; ++k) {
+      for (int t = sub_start[k]; t < sub_start[k + 1];) {
+        int m = 1;
+        while (m < 8 && t + m < sub_start[k + 1] && sub_dl[t + m] == sub_dl[t] - m) ++m;
+        run_dl0.push_back(sub_dl[t]);
+        run_len.push_back(m);
+        run_wo.push_back(t);
+        t += m;
+      }
+      run_start[k + 1] = (int32_t)run_dl0.size();
     }
     // 4. boundary rows: packed offsets from the row's bounding-box corner and
     //    a distinct anchor position (one of the row's own entries) for v
@@ -922,15 +948,20 @@ struct Chain : public ChainBase {
           smin[k] = std::min(smin[k], (long)sub_dl[t]);
           smax[k] = std::max(smax[k], (long)sub_dl[t]);
         }
-      DR_REQUIRE(wcls.size() * 64 == vg.size() && vg.size() % kThreads == 0, DR_ELOGIC, "W stencil: voxel padding");
-      for (size_t t = 0; t < vg.size(); ++t) {
+      DR_REQUIRE(wcls.size() * 64 == gub.size() && gub.size() % kThreads == 0, DR_ELOGIC,
+                 "W stencil: voxel group padding");
+      // a group's window for a run reads lattice indices ub + dl0 + 3 - q,
+      // q in [0, len + 2]: within [ub + dl_min, ub + dl_max + 3]
+      for (size_t t = 0; t < gub.size(); ++t) {
         const int k = wcls[t / 64];
         DR_REQUIRE(k >= 0 && k < ncls, DR_ELOGIC, "W stencil: residue class out of range");
         if (sub_start[k + 1] > sub_start[k])
-          DR_REQUIRE(vub[t] + smin[k] >= 0 && vub[t] + smax[k] < nVl, DR_ELOGIC,
+          DR_REQUIRE(gub[t] + smin[k] >= 0 && gub[t] + smax[k] + 3 < nVl + 4, DR_ELOGIC,
                      "W stencil: sub-stencil read outside the lattice grid");
-        DR_REQUIRE(vg[t] < nreg, DR_ELOGIC, "W stencil: voxel outside the region");
+        for (int p = 0; p < 4; ++p) DR_REQUIRE(gg[4 * t + p] < nreg, DR_ELOGIC, "W stencil: voxel outside the region");
       }
+      for (size_t r = 0; r < run_dl0.size(); ++r)
+        DR_REQUIRE(run_len[r] >= 1 && run_len[r] <= 8, DR_ELOGIC, "W stencil: sub-stencil run length");
     }
     for (size_t k = 0; k < brow.size(); ++k) {
       DR_REQUIRE(banchor[k] < nreg, DR_ELOGIC, "W stencil: anchor outside the region");
@@ -951,7 +982,7 @@ struct Chain : public ChainBase {
     ws.n_bnd = (int)brow.size();
     ws.n_touch = (int)tg.size();
     ws.dc = (int)(15 + 15 * R0 + 15 * R01);
-    ws.nvpad = (long)vg.size();
+    ws.nvpad = (long)gub.size();
     upload(ws.irow, irow);
     upload(ws.ibase, ibase);
     upload(ws.ilidx, ilidx);
@@ -960,9 +991,13 @@ struct Chain : public ChainBase {
     upload(ws.sub_start, sub_start);
     upload(ws.sub_dl, sub_dl);
     upload(ws.sub_w, sub_w);
-    upload(ws.vg, vg);
-    upload(ws.vub, vub);
+    upload(ws.vg, gg);
+    upload(ws.vub, gub);
     upload(ws.wcls, wcls);
+    upload(ws.run_start, run_start);
+    upload(ws.run_dl0, run_dl0);
+    upload(ws.run_len, run_len);
+    upload(ws.run_wo, run_wo);
     upload(ws.brow, brow);
     upload(ws.brp, brp);
     upload(ws.bbase, bbase);
@@ -973,7 +1008,7 @@ struct Chain : public ChainBase {
     upload(ws.trp, trp);
     upload(ws.td, tdall);
     upload(ws.tw, twall);
-    ws.Vl.alloc(std::max<long>(nVl, 1));
+    ws.Vl.alloc(nVl + 4);  // + the reads of a group's absent members past the end
     ws.Vl.zero(stream);
     ws.Vb.alloc(std::max<long>(nreg, 1));
     ws.Vb.zero(stream);
@@ -984,7 +1019,7 @@ struct Chain : public ChainBase {
     wstats[3] = nb_real;
     wstats[4] = nnz;
     wstats[5] = (int64_t)tg.size();
-    wstats[6] = (int64_t)vg.size();
+    wstats[6] = (int64_t)gub.size();
     return true;
   }
 
@@ -1106,7 +1141,8 @@ struct Chain : public ChainBase {
       }
       mark();
       const int ti = (int)(ws.nvpad / kThreads), tb = cdiv(ws.n_touch, kThreads / kWtvLanes);
-      const WtvInt<T> wi{ws.vg.p, ws.vub.p, ws.wcls.p, ws.sub_start.p, ws.sub_dl.p, ws.sub_w.p, ws.Vl.p, WTv.p};
+      const WtvInt<T> wi{ws.vg.p, ws.vub.p, ws.wcls.p, ws.run_start.p, ws.run_dl0.p, ws.run_len.p, ws.run_wo.p,
+                         ws.sub_w.p, ws.Vl.p, WTv.p};
       const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p};
       hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), 0, stream, wi, wb, ti);
       DR_HIP(hipGetLastError());

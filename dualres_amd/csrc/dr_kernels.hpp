@@ -391,8 +391,8 @@ template <typename T> struct WmuBnd {
   int R0, R01; T* Vb;
 };
 template <typename T> struct WtvInt {
-  const int* g; const int* ub; const int* wcls; const int* sub_start; const int* sub_dl; const T* sub_w;
-  const T* Vl; T* out;
+  const int* g4; const int* ub; const int* wcls; const int* run_start; const int* run_dl0; const int* run_len;
+  const int* run_wo; const T* sub_w; const T* Vl; T* out;
 };
 template <typename T> struct WtvBnd {
   int n; const int* g; const int* rp; const uint16_t* d; const T* w; int R0, R01, dc; const T* Vb; T* out;
@@ -489,29 +489,41 @@ k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, int gi, const T* __restrict__ R, const T* _
 }
 
 // W^T v of the interior rows at every data voxel (written, not added).  The
-// voxel list is bucketed by residue class and each bucket padded to whole
-// wavefronts, so a wavefront reads one sub-stencil with scalar loads, in
-// batches of 16 independent gathers; padding lanes (g < 0) compute on a
-// valid lattice base and store nothing.
+// voxels are bucketed by residue class, so a wavefront reads one class's
+// sub-stencil with scalar loads.  A lane takes a group of up to 4 voxels at
+// consecutive lattice indices ub .. ub+3: for an x-run of m entries (lattice
+// offsets dl0, dl0 - 1, ..., dl0 - m + 1) the four voxels read the window
+// Vl[ub + dl0 + 3 - q], q = 0 .. m + 2, once (m + 3 gathers instead of 4 m),
+// entry k of voxel p being x[3 - p + k].  Each voxel sums its entries in
+// sub-stencil order.
 template <typename T>
 __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
   const int t = blk * kThreads + threadIdx.x;
   const int cls = __builtin_amdgcn_readfirstlane(a.wcls[t >> 6]);
-  const int s0 = a.sub_start[cls], s1 = a.sub_start[cls + 1];
+  const int r0 = a.run_start[cls], r1 = a.run_start[cls + 1];
   const T* V = a.Vl + a.ub[t];
-  T acc = (T)0;
-  constexpr int B = 16;
-  int s = s0;
-  for (; s + B <= s1; s += B) {
-    T v[B];
+  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+  for (int r = r0; r < r1; ++r) {
+    const int dl0 = a.run_dl0[r], m = a.run_len[r], wo = a.run_wo[r];
+    T x[11];
 #pragma unroll
-    for (int b = 0; b < B; ++b) v[b] = V[a.sub_dl[s + b]];
+    for (int q = 0; q < 11; ++q) x[q] = q < m + 3 ? V[dl0 + 3 - q] : (T)0;
 #pragma unroll
-    for (int b = 0; b < B; ++b) acc += a.sub_w[s + b] * v[b];
+    for (int k = 0; k < 8; ++k) {
+      if (k < m) {
+        const T w = a.sub_w[wo + k];
+        acc[0] += w * x[3 + k];
+        acc[1] += w * x[2 + k];
+        acc[2] += w * x[1 + k];
+        acc[3] += w * x[k];
+      }
+    }
   }
-  for (; s < s1; ++s) acc += a.sub_w[s] * V[a.sub_dl[s]];
-  const int gj = a.g[t];
-  if (gj >= 0) a.out[gj] = acc;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int gj = a.g4[4 * t + p];
+    if (gj >= 0) a.out[gj] = acc[p];
+  }
 }
 
 // W^T v of the boundary rows at the voxels they touch (written to its own
