@@ -124,11 +124,11 @@ def kernel_bytes(name, p, prec_bytes, c):
     ws = c.w_stats()
     if ws["stencil_layout"]:
         ni, nd, nb, eb = ws["interior_rows"], ws["stencil_entries"], ws["boundary_rows"], ws["boundary_entries"]
-        nt, nv = ws["touched_voxels"], ws["padded_voxels"]
+        nt, nv = ws["touched_voxels"], ws["voxel_groups"]
         if name == "W_mu":   # interior: row words, Ys, v, stencil, R; boundary: row words, packed entries
             return (ni * (12 + 2 * s) + nd * (4 + s) + n_h * s) + (nb * (16 + 2 * s) + eb * (2 + s))
-        if name == "Wt_v":   # interior: voxel slots, stencil, lattice v, output; boundary: voxel words, entries
-            return (nv * 8 + nv // 16 + nd * (4 + s) + ni * s + n_h * s) + (nt * (8 + s) + eb * (2 + s) + nb * s)
+        if name == "Wt_v":   # interior: voxel groups, stencil, lattice v, output; boundary: voxel words, entries
+            return (nv * 20 + nv // 4 + nd * (4 + s) + ni * s + n_h * s) + (nt * (8 + s) + eb * (2 + s) + nb * s)
     if name == "W_mu":
         return nnz * (4 + s) + (n_s + 1) * 4 + 2 * n_s * s + n_h * s
     if name == "Wt_v":

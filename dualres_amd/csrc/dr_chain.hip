@@ -657,6 +657,7 @@ struct Chain : public ChainBase {
   bool build_stencil(const std::vector<int32_t>& rp, const std::vector<int32_t>& cols, const std::vector<T>& vals) {
     const long nnz = rp[n_s];
     const long R0 = reg[0], R01 = (long)reg[0] * reg[1];
+    if (R01 >= (1L << 24)) return false;  // packed offsets unpack with 24-bit multiplies
     auto xyz = [&](long r, long* v) {
       v[0] = r % R0;
       v[1] = (r / R0) % reg[1];

@@ -366,8 +366,10 @@ k_wtv(int n_h, const int* __restrict__ rp, const int* __restrict__ rows, const T
 // entries with 3D offsets packed in 15 bits (5 per axis).
 
 // packed 3D offset (dx | dy << 5 | dz << 10) -> region-grid flat offset
+// (R0 and R01 below 2^24: 24-bit multiplies run at full rate, 32-bit ones at
+// a quarter)
 __device__ __forceinline__ int unpack3(unsigned q, int R0, int R01) {
-  return (int)(q & 31u) + R0 * (int)((q >> 5) & 31u) + R01 * (int)(q >> 10);
+  return (int)((q & 31u) + __umul24((unsigned)R0, (q >> 5) & 31u) + __umul24((unsigned)R01, q >> 10));
 }
 
 // four packed entries: offsets (4 x 16 bits) and weights, read as one vector each

@@ -104,7 +104,7 @@ class Chain:
         L.check(self._lib.dr_w_stats(self._h, L.iptr64(st)))
         return {"stencil_layout": bool(st[0]), "interior_rows": int(st[1]), "stencil_entries": int(st[2]),
                 "boundary_rows": int(st[3]), "boundary_entries": int(st[4]), "nnz": int(st[5]),
-                "touched_voxels": int(st[6]), "padded_voxels": int(st[7])}
+                "touched_voxels": int(st[6]), "voxel_groups": int(st[7])}
 
     def add_data_to_mu(self):
         L.check(self._lib.dr_add_data_to_mu(self._h))
