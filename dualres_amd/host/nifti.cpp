@@ -177,14 +177,15 @@ Image read_nifti(const std::string& path) {
   return im;
 }
 
-void write_nifti(const std::string& path, const Image& like, const std::vector<double>& values) {
+void write_nifti(const std::string& path, const Image& like, const std::vector<double>& values, int datatype) {
   if ((long)values.size() != like.nvox()) throw std::logic_error("write_nifti: size mismatch");
+  if (datatype != 16 && datatype != 64) throw std::logic_error("write_nifti: datatype must be float32 or float64");
   Nifti1Header h = like.hdr;
   h.sizeof_hdr = 348;
   h.dim[0] = 3;
   for (int k = 4; k < 8; ++k) h.dim[k] = 1;
-  h.datatype = 16;
-  h.bitpix = 32;
+  h.datatype = (int16_t)datatype;
+  h.bitpix = datatype == 64 ? 64 : 32;
   h.vox_offset = 352.0f;
   h.scl_slope = 0.0f;
   h.scl_inter = 0.0f;
@@ -194,17 +195,27 @@ void write_nifti(const std::string& path, const Image& like, const std::vector<d
   GzFile g(path, gz ? "wb6" : "wbT");
   if (!g.f) throw std::runtime_error("cannot write " + path);
   const char ext[4] = {0, 0, 0, 0};
-  std::vector<float> f(values.begin(), values.end());
+  std::vector<float> f;
+  if (datatype == 16) f.assign(values.begin(), values.end());
   if (gzwrite(g.f, &h, sizeof(h)) != (int)sizeof(h) || gzwrite(g.f, ext, 4) != 4)
     throw std::runtime_error("write failed: " + path);
   size_t done = 0;
-  const size_t bytes = f.size() * sizeof(float);
-  const char* p = reinterpret_cast<const char*>(f.data());
+  const size_t bytes = datatype == 16 ? f.size() * sizeof(float) : values.size() * sizeof(double);
+  const char* p = datatype == 16 ? reinterpret_cast<const char*>(f.data()) : reinterpret_cast<const char*>(values.data());
   while (done < bytes) {
     const unsigned chunk = (unsigned)std::min<size_t>(bytes - done, 1u << 30);
     if (gzwrite(g.f, p + done, chunk) != (int)chunk) throw std::runtime_error("write failed: " + path);
     done += chunk;
   }
+}
+
+std::string nifti_basename(const std::string& path) {
+  static const char* ext[] = {".nii.gz", ".hdr.gz", ".img.gz", ".nii", ".hdr", ".img"};
+  for (const char* e : ext) {
+    const size_t n = std::strlen(e);
+    if (path.size() > n && path.compare(path.size() - n, n, e) == 0) return path.substr(0, path.size() - n);
+  }
+  return path;
 }
 
 bool same_orientation(const Image& a, const Image& b, float tol) {

@@ -52,8 +52,11 @@ struct Image {
 
 bool is_nifti_file(const std::string& path);
 Image read_nifti(const std::string& path);
-// Writes `values` (nvox, voxel order) with the geometry of `like`, float32.
-void write_nifti(const std::string& path, const Image& like, const std::vector<double>& values);
+// Writes `values` (nvox, voxel order) with the geometry of `like`, as float32
+// (datatype 16) or float64 (datatype 64).
+void write_nifti(const std::string& path, const Image& like, const std::vector<double>& values, int datatype = 16);
+// nifti_makebasename: the path without a .nii/.hdr/.img extension (and .gz)
+std::string nifti_basename(const std::string& path);
 
 inline bool present(double v) { return !(v != v || v == 0.0); }
 

@@ -1,14 +1,16 @@
 """End-to-end runs of the C++ command-line tools on the device: estimate_rbf
 (src/estimate_rbf.cpp) and dualgpm/dualgpmf (src/dualgpm.cpp) on small
 synthetic NIfTI images."""
+import math
 import os
 import re
+import struct
 import subprocess
 
 import numpy as np
 import pytest
 
-from dualres_amd import mce, problem
+from dualres_amd import kriging, mce, problem
 from tests.nifti_io import read_nifti, write_nifti
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -102,3 +104,79 @@ def test_dualgpm_cli_single_resolution_with_estimated_covariance(tmp_path, gpu_a
     mean, _ = read_nifti(base + "_posterior_mean.nii.gz")
     mask = np.transpose(p["mask"], (2, 1, 0))
     assert np.all(np.isfinite(mean[mask]))
+
+
+def _datatype(path):
+    with open(path, "rb") as f:
+        return struct.unpack_from("<h", f.read(72), 70)[0]
+
+
+def _kriging_inputs(tmp_path, dtype=np.float64):
+    p = problem.dual_problem((24, 20, 16), 1.0, THETA)
+    src, space = tmp_path / "src.nii.gz", tmp_path / "space.nii"
+    write_nifti(str(src), np.transpose(p["hr"], (2, 1, 0)), pixdim=(1.0, 1.0, 1.0),
+                qoffset=tuple(p["q_hr"][:3, 3]), dtype=dtype)
+    write_nifti(str(space), np.transpose(p["std"], (2, 1, 0)), pixdim=(2.0, 2.0, 2.0),
+                qoffset=tuple(p["q_std"][:3, 3]), dtype=dtype)
+    fwhm, nu = 6.0, 1.99999
+    theta = (1.0, math.log(2.0) / (fwhm / 2) ** nu, nu)  # kernels.inl:45-50
+    W = kriging.kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], theta, 3 * 2.0)
+    return p, src, space, W
+
+
+@pytest.mark.gpu
+def test_write_kriging_matrix_cli(tmp_path, gpu_available):
+    p, src, space, W = _kriging_inputs(tmp_path)
+    r = subprocess.run([os.path.join(BIN, "write_kriging_matrix"), str(src), "--space", str(space)],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "W.csv written" in r.stdout
+    lines = (tmp_path / "W.csv").read_text().splitlines()
+    assert lines[0] == "Data,ColumnIndex,RowCounts,NRow,Ncol"
+    rows = [ln.split(",") for ln in lines[1:]]
+    assert len(rows) == W.nnz and all(len(f) == 5 for f in rows)
+    assert (int(rows[0][3]), int(rows[0][4])) == W.shape
+    assert all(f[3] == "" and f[4] == "" for f in rows[1:])
+    vals = np.array([float(f[0]) for f in rows])
+    np.testing.assert_array_equal([int(f[1]) for f in rows], W.indices)
+    nrc = min(W.nnz, W.shape[0] + 1)
+    np.testing.assert_array_equal([int(f[2]) for f in rows[:nrc]], W.indptr[:nrc])
+    assert all(f[2] == "" for f in rows[nrc:])
+    # values printed as float with 6 significant digits
+    np.testing.assert_allclose(vals, W.data, rtol=6e-6, atol=1e-30)
+
+
+@pytest.mark.gpu
+def test_interpolate_cli(tmp_path, gpu_available):
+    p, src, space, W = _kriging_inputs(tmp_path)
+    r = _run([os.path.join(BIN, "interpolate"), str(src), "--space", str(space)])
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "src_6mm_fwhm_kriged.nii"
+    vol, q = read_nifti(str(out))
+    assert _datatype(out) == 64  # the space image's data type
+    np.testing.assert_allclose(q, p["q_std"], atol=1e-6)
+    sp = np.transpose(p["std"], (2, 1, 0))
+    inside = sp != 0
+    want = W @ p["Yh"]
+    # the space image's nonzero voxels in voxel order (x fastest) receive W Y
+    got = np.transpose(vol, (2, 1, 0)).ravel()[np.flatnonzero(p["std"].ravel() != 0)]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert np.all(vol[~inside] == 0)
+
+
+@pytest.mark.gpu
+def test_interpolate_reproduces_constants(tmp_path, gpu_available):
+    """Kriging weights are normalised to sum to one, so a constant image is
+    interpolated exactly wherever a row has any neighbour."""
+    p, src, space, W = _kriging_inputs(tmp_path, dtype=np.float32)
+    const = np.where(p["mask"], 3.25, 0.0)
+    write_nifti(str(src), np.transpose(const, (2, 1, 0)), pixdim=(1.0, 1.0, 1.0), qoffset=tuple(p["q_hr"][:3, 3]))
+    r = _run([os.path.join(BIN, "interpolate"), str(src), "--space", str(space), "--fwhm", "4.5"])
+    assert r.returncode == 0, r.stderr
+    vol, _ = read_nifti(str(tmp_path / "src_4mm_fwhm_kriged.nii"))
+    assert _datatype(tmp_path / "src_4mm_fwhm_kriged.nii") == 16
+    got = np.transpose(vol, (2, 1, 0)).ravel()[np.flatnonzero(p["std"].ravel() != 0)]
+    has = np.diff(W.indptr) > 0
+    assert has.sum() > 0.9 * len(has)
+    np.testing.assert_allclose(got[has], 3.25, rtol=2e-7)
+    assert np.all(got[~has] == 0)

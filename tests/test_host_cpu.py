@@ -181,3 +181,24 @@ def test_cli_usage_errors():
         assert h.returncode == 0
     r = subprocess.run([os.path.join(BIN, "dualgpm"), "--highres", "/nonexistent.nii"], capture_output=True, text=True)
     assert r.returncode == 1
+
+
+def test_kriging_cli_usage_errors(tmp_path):
+    """KrigingCommandParser.inl:66-176: usage, help, missing inputs, bad values."""
+    img = tmp_path / "img.nii"
+    write_nifti(str(img), np.ones((4, 4, 4), np.float32))
+    for name in ("interpolate", "write_kriging_matrix"):
+        exe = os.path.join(BIN, name)
+        r = subprocess.run([exe], capture_output=True, text=True)
+        assert r.returncode == 1 and "Usage" in r.stderr
+        assert subprocess.run([exe, "--help"], capture_output=True, text=True).returncode == 0
+        r = subprocess.run([exe, str(img)], capture_output=True, text=True)
+        assert r.returncode == 1 and "must supply a --space" in r.stderr
+        r = subprocess.run([exe, "--space", str(img)], capture_output=True, text=True)
+        assert r.returncode == 1 and "must supply a NIfTI file input" in r.stderr
+        r = subprocess.run([exe, str(img), "--space", str(img), "--exponent", "2.5"], capture_output=True, text=True)
+        assert r.returncode == 1 and "bad parameter value (exponent" in r.stderr
+        r = subprocess.run([exe, str(img), "--space", str(tmp_path / "missing.nii")], capture_output=True, text=True)
+        assert r.returncode == 1 and "not a NIfTI file" in r.stderr
+        r = subprocess.run([exe, str(img), "--bogus"], capture_output=True, text=True)
+        assert r.returncode == 1 and "Unrecognized option" in r.stderr
