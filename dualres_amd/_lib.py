@@ -82,6 +82,9 @@ SIGNATURES = {
     "dr_get_mu_data": [_P, _D, C.c_int64],
     "dr_set_sigma_sq_inv": [_P, _D],
     "dr_get_sigma_sq_inv": [_P, _D],
+    "dr_set_tau_sq_inv": [_P, C.c_double],
+    "dr_get_tau_sq_inv": [_P, _D],
+    "dr_mu_data_sumsq": [_P, _D],
     "dr_update": [_P, C.c_double, C.c_int, C.c_double, C.POINTER(dr_update_result)],
     "dr_update_with_noise": [_P, C.c_double, C.c_int, C.c_double, _D, _D, C.POINTER(dr_update_result)],
     "dr_log_prior": [_P, _D],

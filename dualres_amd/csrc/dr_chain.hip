@@ -145,6 +145,8 @@ struct ChainBase {
   // cached energies of the current state A
   bool stats_valid = false;
   double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
+  // prior precision tau^-2 (1 / theta[0] until the caller updates it)
+  double tau_sq_inv = 0.0;
   // profiling
   bool prof = false;
   double prof_ms = 0.0, prof_fft_ms = 0.0, prof_w_ms = 0.0;
@@ -168,6 +170,8 @@ struct ChainBase {
   virtual void get_state(double* mr, double* mi, double* pr, double* pi) = 0;
   virtual void gradient(double* gr, double* gi) = 0;
   virtual void refresh_stats() = 0;
+  virtual void set_tau_sq_inv(double t) = 0;
+  virtual double mu_data_sumsq() = 0;
   virtual void get_lambda(double* out) = 0;
   virtual void reset_kernel_profile() = 0;
   virtual void kernel_profile(double* ms, int max_k, int* n_k, int64_t* n_steps) = 0;
@@ -192,6 +196,8 @@ struct Chain : public ChainBase {
   DevBuf<cx<T>> muA, muB, p, miA, miB, pim, T1, T2;
   DevBuf<T> invlam;
   DevBuf<double> lam;  // Re lambda on the half spectrum (float64)
+  DevBuf<T> invlam0;    // 1/lambda and lambda at theta[0], kept once tau^-2 is updated
+  DevBuf<double> lam0;
   DevBuf<T> RA, RB, Yg, WTv, WTvb, Rf, zbuf;
   DevBuf<uint8_t> mask;
   DevBuf<int> Wrp, Wcols, WTrp, WTrows, gI, outg;
@@ -316,6 +322,7 @@ struct Chain : public ChainBase {
     H0 = M[0] / 2 + 1;
     N = (long)M[0] * M[1] * M[2];
     Nh = (long)H0 * M[1] * M[2];
+    tau_sq_inv = 1.0 / cfg.theta[0];  // MultiResParameters.h:256
     for (int a = 0; a < 3; ++a) {
       const int len = M[a];
       const int flen = a == 0 ? len / 2 : len;
@@ -1381,6 +1388,40 @@ struct Chain : public ChainBase {
     stats_valid = true;
   }
 
+  // _update_tau_sq_inv (MultiResParameters.h:863-872): lambda *= t / tau^-2.
+  // lambda and 1/lambda are rescaled from their values at theta[0], so
+  // repeated updates do not accumulate rounding.
+  void set_tau_sq_inv(double t) override {
+    DR_REQUIRE(t > 0.0 && std::isfinite(t), DR_EDOMAIN, "tau^-2 must be positive and finite");
+    if (!invlam0.p) {
+      invlam0.alloc(Nh);
+      lam0.alloc(Nh);
+      DR_HIP(hipMemcpyAsync(invlam0.p, invlam.p, Nh * sizeof(T), hipMemcpyDeviceToDevice, stream));
+      DR_HIP(hipMemcpyAsync(lam0.p, lam.p, Nh * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    }
+    const double t0 = 1.0 / cfg.theta[0];
+    hipLaunchKernelGGL(k_rescale_spectrum<T>, dim3(cdiv(Nh, kThreads)), dim3(kThreads), 0, stream, (long)Nh,
+                       invlam0.p, lam0.p, invlam.p, lam.p, (T)(t0 / t), t / t0);
+    DR_HIP(hipGetLastError());
+    tau_sq_inv = t;
+    stats_valid = false;
+  }
+  // _real_mu.squaredNorm() of the current state (:866-867): sum of Re(mu)^2
+  // over the data voxels, from the region field of state A
+  double mu_data_sumsq() override {
+    refresh_stats();
+    clear_slots();
+    const int g = (int)std::min<long>(cdiv(nreg, kThreads), 2048);
+    note_blocks(S_TMP, g);
+    hipLaunchKernelGGL(k_sumsq_masked<T>, dim3(g), dim3(kThreads), 0, stream, (long)nreg, RA.p, mask.p,
+                       pslot(S_TMP));
+    DR_HIP(hipGetLastError());
+    finalize();
+    double h[NSLOT];
+    read_sums(h);
+    return h[S_TMP];
+  }
+
   // ------------------------------------------------------------ test hooks
   void upload_real(const double* src, DevBuf<T>& dst) {
     if (dst.n != (size_t)N) dst.alloc(N);
@@ -1669,6 +1710,19 @@ int dr_get_sigma_sq_inv(dr_chain* c, double* s) {
   return guarded([&] {
     auto& k = *c->impl;
     for (int i = 0; i < k.cfg.n_datasets; ++i) s[i] = k.sigma[i];
+  });
+}
+int dr_set_tau_sq_inv(dr_chain* c, double tau_sq_inv) {
+  return guarded([&] { c->impl->set_tau_sq_inv(tau_sq_inv); });
+}
+int dr_get_tau_sq_inv(dr_chain* c, double* tau_sq_inv) {
+  return guarded([&] { *tau_sq_inv = c->impl->tau_sq_inv; });
+}
+int dr_mu_data_sumsq(dr_chain* c, double* out) {
+  return guarded([&] {
+    auto& k = *c->impl;
+    DR_REQUIRE(k.data_set, DR_ELOGIC, "dr_mu_data_sumsq: data not set");
+    *out = k.mu_data_sumsq();
   });
 }
 

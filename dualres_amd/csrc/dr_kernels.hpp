@@ -638,6 +638,29 @@ k_sse_h(long n, const T* __restrict__ R, const T* __restrict__ Yg, const uint8_t
   red_write(acc, red, part, blockIdx.x);
 }
 
+// sum of R^2 over the data voxels (mask) of a region field
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_sumsq_masked(long n, const T* __restrict__ R, const uint8_t* __restrict__ mask, double* part) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (long g = (long)blockIdx.x * kThreads + threadIdx.x; g < n; g += (long)gridDim.x * kThreads)
+    if (mask[g]) acc += (double)R[g] * (double)R[g];
+  red_write(acc, red, part, blockIdx.x);
+}
+
+// lambda and 1/lambda rescaled from their base values (the tau^-2 update)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_rescale_spectrum(long n, const T* __restrict__ il0, const double* __restrict__ l0, T* __restrict__ il,
+                   double* __restrict__ l, T s_il, double s_l) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    il[i] = il0[i] * s_il;
+    l[i] = l0[i] * s_l;
+  }
+}
+
 struct FinalizeArgs { int nslot; int nblk[32]; long pstride; };
 __global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a, const double* part, double* sums);
 

@@ -147,6 +147,21 @@ class Chain:
         L.check(self._lib.dr_get_sigma_sq_inv(self._h, L.dptr(s)))
         return s[: self.n_datasets]
 
+    def set_tau_sq_inv(self, t):
+        """_update_tau_sq_inv's assignment: lambda *= t / (previous tau^-2)."""
+        L.check(self._lib.dr_set_tau_sq_inv(self._h, float(t)))
+
+    def tau_sq_inv(self):
+        t = C.c_double()
+        L.check(self._lib.dr_get_tau_sq_inv(self._h, C.byref(t)))
+        return t.value
+
+    def mu_data_sumsq(self):
+        """Sum of Re(mu)^2 over the data voxels of the current state."""
+        s = C.c_double()
+        L.check(self._lib.dr_mu_data_sumsq(self._h, C.byref(s)))
+        return s.value
+
     def lambda_(self):
         out = np.empty(self.N)
         L.check(self._lib.dr_get_lambda(self._h, L.dptr(out)))

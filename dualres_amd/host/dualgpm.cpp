@@ -41,7 +41,7 @@ struct Options {
   double mhtarget = 0.65;
   unsigned seed = 1;
   int threads = 0;
-  bool monitor = false, samples = false, rq = false, help = false, error = false;
+  bool monitor = false, samples = false, rq = false, help = false, error = false, update_tau = false;
   std::string embedding = "pow2", state = "compat";
   int device = 0;
 };
@@ -63,6 +63,7 @@ void help(const std::string& caller) {
             << "  --burnin n  --nsave/-n n  --thin n  --leapfrog/-L n  --mhtarget f\n"
             << "  --seed n  --threads n  --output/-o base  --samples  --monitor  --debug\n"
             << "  --rational-quadratic\n"
+            << "  --update-tau                Gibbs-update the prior variance (compiled out of the reference)\n"
             << "  --embedding pow2|2x|smooth  circulant embedding rule (default pow2)\n"
             << "  --state compat|real         complex-state energies (default) or real state\n"
             << "  --device n                  GPU ordinal\n"
@@ -110,6 +111,7 @@ Options parse(int argc, char** argv) {
       else if (a == "--output" || a == "-o") { if (need(i, 1, "\nWarning: --output option requires 1 string argument\n")) o.output_base = argv[++i]; }
       else if (a == "--omask" || a == "-om") { if (need(i, 1, "\nWarning: --omask option requires one argument\n")) o.omask = argv[++i]; }
       else if (a == "--rational-quadratic") o.rq = true;
+      else if (a == "--update-tau") o.update_tau = true;
       else if (a == "--samples") o.samples = true;
       else if (a == "--seed") { if (need(i, 1, "\nWarning: --seed option requires 1 integer argument\n")) o.seed = (unsigned)std::abs(std::stoi(argv[++i])); }
       else if (a == "--smask" || a == "-sm") { if (need(i, 1, "\nWarning: --smask option requires one argument\n")) o.smask = argv[++i]; }
@@ -272,6 +274,7 @@ int main(int argc, char* argv[]) {
     fo.seed = in.seed;
     fo.monitor = in.monitor;
     fo.output_samples = in.samples;
+    fo.update_tau = in.update_tau;
     if (in.embedding != "pow2") {
       const BBox bb = bounding_box(hmask);
       const auto ijk = nonzero_indices_bounded_by_box(hr, bb);

@@ -91,6 +91,17 @@ class Chain {
     n_out_ = (long)ijk.size();
   }
   void set_sigma_sq_inv(const std::vector<double>& s) { check(dr_set_sigma_sq_inv(h_, s.data())); }
+  void set_tau_sq_inv(double t) { check(dr_set_tau_sq_inv(h_, t)); }
+  double tau_sq_inv() {
+    double t = 0;
+    check(dr_get_tau_sq_inv(h_, &t));
+    return t;
+  }
+  double mu_data_sumsq() {
+    double s = 0;
+    check(dr_mu_data_sumsq(h_, &s));
+    return s;
+  }
   dr_update_result update(double eps, int L, double u) {
     dr_update_result r{};
     check(dr_update(h_, eps, L, u, &r));

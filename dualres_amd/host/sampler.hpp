@@ -164,6 +164,9 @@ struct FitOptions {
   bool monitor = false;
   bool output_samples = false;
   bool quiet = false;  // no progress messages (served fits)
+  // Gibbs update of tau^-2 after the sigma draws (MultiResParameters.h:723-725),
+  // which both reference binaries compile out (DUALRES_DONT_UPDATE_TAU)
+  bool update_tau = false;
 };
 
 struct MultiResData {  // MultiResData.h:21-127
@@ -245,6 +248,13 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
       }
       chain.set_sigma_sq_inv(sig);
     }
+    // _update_tau_sq_inv (MultiResParameters.h:863-872)
+    if (opt.update_tau) {
+      const T shape = (T)(0.5 * data.Yh.size() + 1);
+      const T rate = (T)(0.5 * chain.mu_data_sumsq());
+      std::gamma_distribution<T> G(shape, (T)(1 / rate));
+      chain.set_tau_sq_inv(G(rng()));
+    }
     hmc.update((T)r.mh_rate);
     if (hmc.save_iteration()) {
       if (save_count == 0) start = std::chrono::high_resolution_clock::now();
@@ -257,7 +267,7 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
         const std::vector<double> mu = chain.mu_();
         for (double v : mu) samples_out << (T)v << "\t";
         for (int i = 0; i < data.n_datasets; ++i) samples_out << sigma[i] << "\t";
-        samples_out << (T)std::sqrt(data.theta[0]) << "\t" << log_lik;
+        samples_out << (T)std::sqrt(1.0 / chain.tau_sq_inv()) << "\t" << log_lik;
         if (save_count < hmc.n_save() - 1) samples_out << std::endl;
       }
       ++save_count;

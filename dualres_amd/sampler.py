@@ -174,6 +174,14 @@ def gibbs_sigma(chain, rng, n_h, n_s, sse_h, sse_s):
     return s
 
 
+def gibbs_tau(chain, rng, n_h):
+    """_update_tau_sq_inv (MultiResParameters.h:863-872): Gamma(n_h/2 + 1,
+    rate = sum mu^2 / 2) over the data voxels; lambda *= new / old tau^-2."""
+    t = rng.gamma(0.5 * n_h + 1, 1.0 / (0.5 * chain.mu_data_sumsq()))
+    chain.set_tau_sq_inv(t)
+    return t
+
+
 def init_sigma(chain, rng, tau2):
     """_initialize_sigma (MultiResParameters.h:764-778)."""
     s = [rng.gamma(tau2, 1.0)]
@@ -183,7 +191,7 @@ def init_sigma(chain, rng, tau2):
     return s
 
 
-def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None):
+def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None, update_tau=False):
     """fit_model loop (gaussian_process_model.h:134-171) on a prepared chain.
 
     The chain must have its data, mu (with data added) and output indices
@@ -203,6 +211,8 @@ def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None):
         grads += L + 1
         hmc.update(r["mh_rate"])
         s = gibbs_sigma(chain, rng, chain.n_h, n_s, r["sse_h"], r["sse_s"])
+        if update_tau:
+            gibbs_tau(chain, rng, chain.n_h)
         if hmc.save_iteration():
             if save_count == 0:
                 t_start = time.perf_counter()

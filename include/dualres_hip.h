@@ -146,6 +146,18 @@ int dr_get_mu_data(dr_chain* c, double* out, int64_t n_h);
 int dr_set_sigma_sq_inv(dr_chain* c, const double* s);
 int dr_get_sigma_sq_inv(dr_chain* c, double* s);
 
+/* Prior precision tau^-2 (_update_tau_sq_inv, MultiResParameters.h:863-872,
+ * compiled out of both reference binaries by DUALRES_DONT_UPDATE_TAU,
+ * src/CMakeLists.txt:44,59).  Setting it rescales lambda by
+ * tau_sq_inv / (previous tau^-2), the reference's direction; it starts at
+ * 1 / theta[0].  The host draws the new value, Gamma(n_h/2 + 1, rate
+ * dr_mu_data_sumsq / 2), after the sigma draws.  tau() = sqrt(1 / tau^-2). */
+int dr_set_tau_sq_inv(dr_chain* c, double tau_sq_inv);
+int dr_get_tau_sq_inv(dr_chain* c, double* tau_sq_inv);
+/* _real_mu.squaredNorm() (:866-867): sum of Re(mu)^2 over the data voxels
+ * of the current state. */
+int dr_mu_data_sumsq(dr_chain* c, double* out);
+
 /* One HMC iteration (MultiResParameters::update, :673-727) with momentum
  * normals from the device Philox stream and the host-drawn accept uniform
  * u_accept in [0,1).  Does not perform the Gibbs draws. */

@@ -242,8 +242,11 @@ class MultiResParameters:
             momentum = (momentum + k * eps * self.compute_gradient(data, mu_star)).astype(self.C)
         return mu_star, momentum
 
-    def update(self, data, eps, L):
-        """One HMC iteration  (MultiResParameters.h:673-727).  Returns min(1, R)."""
+    def update(self, data, eps, L, update_tau=False):
+        """One HMC iteration  (MultiResParameters.h:673-727).  Returns min(1, R).
+
+        ``update_tau`` compiles in the tau^-2 draw that both reference
+        binaries leave out (DUALRES_DONT_UPDATE_TAU, :723-725)."""
         mu_star = self.mu.copy()
         self._compute_mass_matrix_eigen_values()
         e0 = self._sample_momentum()
@@ -261,7 +264,21 @@ class MultiResParameters:
             self.mu = mu_star
         self.real_mu = self._gather_re(self.mu)
         self._update_sigma_sq_inv(data)
+        if update_tau:
+            self._update_tau_sq_inv()
         return min(1.0, R)
+
+    def _update_tau_sq_inv(self):
+        """Gibbs draw of tau^-2  (MultiResParameters.h:863-872), then
+        ``_lambda *= tau_sq_inv_star / _tau_sq_inv`` in the reference's direction."""
+        shape = self.T(0.5) * self.T(self.real_mu.size) + self.T(1)
+        rate = self.T(0.5) * self.T(np.dot(self.real_mu, self.real_mu))
+        self.set_tau_sq_inv(self.T(self.rng.gamma(shape, 1.0 / rate)))
+
+    def set_tau_sq_inv(self, t):
+        """The assignment half of _update_tau_sq_inv (:870-871)."""
+        self.lam = (self.lam * (self.T(t) / self.tau_sq_inv)).astype(self.C)
+        self.tau_sq_inv = self.T(t)
 
     def _update_sigma_sq_inv(self, data):
         """Gibbs draws  (MultiResParameters.h:820-859)."""

@@ -285,3 +285,48 @@ def test_fp32_chain_tracks_fp64(gpu_available):
     assert rel(summ["fp32"].mode_mu(), summ["fp64"].mode_mu()) < 1e-3
     for c in chains.values():
         c.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tau_update_matches_oracle(gpu_available, precision):
+    """dr_set_tau_sq_inv rescales lambda as _update_tau_sq_inv does
+    (MultiResParameters.h:870-871); eigenvalues, gradient and prior then match
+    the oracle after the same assignment.  dr_mu_data_sumsq is the rate input
+    |Re mu at I|^2 (:866-867).  Repeated assignments rescale from the base
+    spectrum, so returning to 1/theta0 restores lambda bit for bit."""
+    p = small_dual()
+    o, data, c, mu, _ = _setup(p, THETA2, precision, "compat")
+    lam0 = c.lambda_()
+    assert c.tau_sq_inv() == pytest.approx(1 / THETA2[0], rel=1e-15)
+    s = c.mu_data_sumsq()
+    assert s == pytest.approx(float(np.sum(c.mu() ** 2)), rel=1e-12 if precision == "fp64" else 1e-6)
+    assert s == pytest.approx(float(np.sum(o._gather_re(o.mu) ** 2)), rel=1e-12 if precision == "fp64" else 1e-6)
+    for t in (0.4, 2.9):
+        c.set_tau_sq_inv(t)
+        o.set_tau_sq_inv(t)
+        tol = 1e-12 if precision == "fp64" else 1e-6
+        assert rel(c.lambda_(), o.lam.real) < tol
+        g_e, g_o = c.gradient(), o.compute_gradient(data, o.mu)
+        assert rel(g_e.real, g_o.real) < (1e-10 if precision == "fp64" else 2e-4)
+        assert c.log_prior() == pytest.approx(o.log_prior(), rel=1e-9 if precision == "fp64" else 1e-4)
+    c.set_tau_sq_inv(1 / THETA2[0])
+    assert np.array_equal(c.lambda_(), lam0)
+    with pytest.raises(ValueError):
+        c.set_tau_sq_inv(-1.0)
+    c.close()
+
+
+def test_fit_with_tau_update(gpu_available):
+    """fit(update_tau=True): the tau^-2 draw follows the sigma draws each
+    iteration; the chain stays finite and tau moves away from theta0."""
+    from dualres_amd.sampler import HMCParameters, fit
+    p = small_dual(dims=(16, 14, 12))
+    c = engine_chain(p, THETA2, precision="fp32", state_mode="compat", seed=5)
+    c.add_data_to_mu()
+    c.set_output_indices(p["ijk"])
+    hmc = HMCParameters(burnin=10, n_save=10, thin=1, integrator_steps=6, starting_eps=0.1, mh_target=0.65)
+    summ, st = fit(c, hmc, THETA2[0], n_s=p["Ys"].size, seed=4, update_tau=True)
+    t = c.tau_sq_inv()
+    assert np.isfinite(t) and t > 0 and abs(t - 1 / THETA2[0]) > 1e-6
+    assert np.all(np.isfinite(summ.mode_mu()))
+    c.close()

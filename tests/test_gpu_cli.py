@@ -180,3 +180,25 @@ def test_interpolate_reproduces_constants(tmp_path, gpu_available):
     assert has.sum() > 0.9 * len(has)
     np.testing.assert_allclose(got[has], 3.25, rtol=2e-7)
     assert np.all(got[~has] == 0)
+
+
+@pytest.mark.gpu
+def test_dualgpm_cli_update_tau(tmp_path, gpu_available):
+    """--update-tau compiles in the reference's tau^-2 Gibbs step
+    (MultiResParameters.h:723-725): the tau column of the samples file moves;
+    without the flag it stays at sqrt(theta0)."""
+    p, hr, sr = _dual_inputs(tmp_path)
+    taus = {}
+    for flag in (False, True):
+        base = str(tmp_path / f"fit_{int(flag)}")
+        args = [os.path.join(BIN, "dualgpmf"), "--highres", str(hr), "--stdres", str(sr), "--covariance",
+                *map(str, THETA), "--burnin", "20", "--nsave", "8", "--thin", "1", "--leapfrog", "8", "--seed", "3",
+                "--samples", "--output", base] + (["--update-tau"] if flag else [])
+        r = _run(args)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        rows = [ln.split("\t") for ln in open(base + "_samples.dat").read().splitlines() if ln.strip()]
+        assert len(rows) == 8
+        taus[flag] = np.array([float(row[-2]) for row in rows])
+    np.testing.assert_allclose(taus[False], np.sqrt(THETA[0]), rtol=1e-6)
+    assert np.all(taus[True] > 0) and np.all(np.isfinite(taus[True]))
+    assert np.ptp(taus[True]) > 0

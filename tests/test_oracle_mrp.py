@@ -140,3 +140,39 @@ def test_update_accept_and_energies_consistent():
     # E0 = -1/2 sum |z|^2 is the kinetic energy of the freshly drawn momentum
     assert o.accepted  # u = 0 < R always unless R = 0
     assert o.sigma_sq_inv == [2.0, 3.0]
+
+
+def test_tau_update_matches_rescaled_covariance():
+    """The reference's tau^-2 assignment (MultiResParameters.h:870-871)
+    multiplies lambda by new / old tau^-2.  The spectrum is linear in the
+    variance parameter, so the updated chain equals a fresh chain built with
+    variance theta0 * (t / tau0^-2): same eigenvalues, gradient and prior.  The
+    Gibbs draw uses shape n/2 + 1 and rate |Re mu at I|^2 / 2 (:866-867)."""
+    o, data = _tiny(dual=True)
+    t = 3.7
+    f = t / o.tau_sq_inv
+    o.set_tau_sq_inv(t)
+    th = (THETA[0] * f, THETA[1], THETA[2])
+    dims = (4, 3, 3)
+    ijk = np.array([[i, j, k] for k in range(3) for j in range(3) for i in range(4) if (i + j + k) % 3 != 1])
+    ref = mrp.MultiResParameters(2, th, ijk, problem.qform_axis(1.0, dims), rng=mrp.NumpyRng(0))
+    ref.sigma_sq_inv = list(o.sigma_sq_inv)
+    ref.mu = o.mu.copy()
+    np.testing.assert_allclose(o.lam, ref.lam, rtol=1e-12, atol=1e-12 * np.abs(ref.lam).max())
+    g, g_ref = o.compute_gradient(data, o.mu), ref.compute_gradient(data, ref.mu)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-10, atol=1e-12 * np.abs(g_ref).max())
+    assert o.log_prior() == pytest.approx(ref.log_prior(), rel=1e-12)
+    assert o.tau() == pytest.approx(1 / np.sqrt(t), rel=1e-15)
+
+    class Capture(mrp.NumpyRng):
+        def gamma(self, shape, scale):
+            self.args = (shape, scale)
+            return 2.5
+
+    o.rng = Capture(1)
+    o.real_mu = o._gather_re(o.mu)
+    o._update_tau_sq_inv()
+    re_at_I = o.mu.reshape(-1)[o.I].real
+    assert o.rng.args[0] == pytest.approx(0.5 * len(o.I) + 1)
+    assert o.rng.args[1] == pytest.approx(2.0 / np.sum(re_at_I ** 2), rel=1e-12)
+    assert o.tau_sq_inv == 2.5
