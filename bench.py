@@ -124,11 +124,13 @@ def kernel_bytes(name, p, prec_bytes, c):
     ws = c.w_stats()
     if ws["stencil_layout"]:
         ni, nd, nb, eb = ws["interior_rows"], ws["stencil_entries"], ws["boundary_rows"], ws["boundary_entries"]
-        nt, nv = ws["touched_voxels"], ws["voxel_groups"]
+        nt = ws["touched_voxels"]
+        ntile, ncell, tvox = ws["interior_tiles"], ws["tile_cells"], ws["tile_voxels"]
         if name == "W_mu":   # interior: row words, Ys, v, stencil, R; boundary: row words, packed entries
             return (ni * (12 + 2 * s) + nd * (4 + s) + n_h * s) + (nb * (16 + 2 * s) + eb * (2 + s))
-        if name == "Wt_v":   # interior: voxel groups, stencil, lattice v, output; boundary: voxel words, entries
-            return (nv * 20 + nv // 4 + nd * (4 + s) + ni * s + n_h * s) + (nt * (8 + s) + eb * (2 + s) + nb * s)
+        if name == "Wt_v":   # interior: per tile its lattice box and voxels (mask in, W^T v out), the stencil;
+            #                   boundary: voxel words, entries
+            return (ntile * (12 + ncell * s + tvox * (1 + s)) + nd * (4 + s)) + (nt * (8 + s) + eb * (2 + s) + nb * s)
     if name == "W_mu":
         return nnz * (4 + s) + (n_s + 1) * 4 + 2 * n_s * s + n_h * s
     if name == "Wt_v":
@@ -175,35 +177,43 @@ def calibration_copy(nbytes=256 * 2 ** 20, reps=3):
     log(f"[bench] calibration copy: {reps} x {nbytes} bytes device to device")
 
 
-def serving_leg(rates=(4.0, 16.0), n_req=24, workers=4, seed=20261202):
+def serving_leg(rates=(4.0, 8.0, 16.0), n_req=32, workers=4, seed=20261202, n_images=32):
     """Mixed fits through the fit server: Poisson arrivals at each rate, half
     single- and half dual-resolution, highres 48^3 or 64x80x64 at 1.5 mm,
     theta = (1, U(0.08, 0.2), U(0.8, 1.8)), 40 burn-in + 40 saved iterations of
-    L = 25 in FP32.  Returns p50/p99 submit-to-result latency per rate."""
+    L = 25 in FP32.  Requests cycle over `n_images` synthetic images, each
+    request with its own HMC seed.  Returns p50/p99 submit-to-result latency
+    per rate."""
     from dualres_amd import problem, serve
     rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(n_req):
+    images = []
+    for i in range(min(n_req, n_images)):
         dims = (48, 48, 48) if i % 4 < 2 else (64, 80, 64)
         th = (1.0, float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 1.8)))
-        kw = dict(burnin=40, nsave=40, thin=1, L=L_STEPS, seed=1000 + i, precision="fp32")
         if i % 2 == 0:
             q = problem.single_problem(dims, 1.5, th, seed_gp=100 + i)
-            reqs.append(serve.make_request(q["hr"], q["q_hr"], th, **kw))
+            images.append((q["hr"], q["q_hr"], th, {}))
         else:
             q = problem.dual_problem(dims, 1.5, th, seed_gp=100 + i)
-            reqs.append(serve.make_request(q["hr"], q["q_hr"], th, std=q["std"], q_std=q["q_std"], **kw))
+            images.append((q["hr"], q["q_hr"], th, {"std": q["std"], "q_std": q["q_std"]}))
+    reqs = []
+    for i in range(n_req):
+        hr, qh, th, extra = images[i % len(images)]
+        reqs.append(serve.make_request(hr, qh, th, burnin=40, nsave=40, thin=1, L=L_STEPS, seed=1000 + i,
+                                       precision="fp32", **extra))
     out = []
     for rate in rates:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
         with serve.Server(200 * 10 ** 9, workers=workers) as srv:
             t0 = time.perf_counter()
             ids = []
-            for r, ta in zip(reqs, arrivals):
+            for k, (r, ta) in enumerate(zip(reqs, arrivals)):
                 dt = ta - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
                 ids.append(srv.submit(r))
+                if (k + 1) % 100 == 0:
+                    log(f"[bench] serving at {rate}/s: {k + 1} of {n_req} submitted")
             res = [srv.wait(i, timeout=600) for i in ids]
             wall = time.perf_counter() - t0
         bad = [x["error"] for x in res if x["status"] != 0]
@@ -286,15 +296,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-serve", action="store_true", help="skip the fit-server latency leg")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests per arrival rate in the serving leg")
+    ap.add_argument("--serve-rates", default="4,8,16", help="Poisson arrival rates (fits/s) of the serving leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the --precision complex-state chain (for profiler runs)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving leg and print its JSON (latency study with --serve-requests)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); chains and requests shard across GPUs "
               f"without communication, so the per-GPU figure is the whole measurement.", file=sys.stderr)
         sys.exit(2)
+    if a.serve_only:
+        print(json.dumps(serving_leg(rates=tuple(float(x) for x in a.serve_rates.split(",")),
+                                     n_req=a.serve_requests)), flush=True)
+        return
     p = build_problem()
     results = {}
     main_chain = None
@@ -329,7 +347,8 @@ def main():
     main_chain.close()
     if a.calib_copy:
         calibration_copy()
-    serving = None if a.main_only or a.no_serve else serving_leg()
+    serving = None if a.main_only or a.no_serve else serving_leg(
+        rates=tuple(float(x) for x in a.serve_rates.split(",")), n_req=a.serve_requests)
     batch = None if a.main_only or a.no_serve else batch_leg()
     cpu = None
     if not a.no_cpu:

@@ -141,7 +141,7 @@ struct ChainBase {
   bool data_set = false;
   bool use_cls = false;    // class-stencil W kernels active
   bool force_csr = false;  // keep the plain CSR kernels (tests)
-  int64_t wstats[7] = {0, 0, 0, 0, 0, 0, 0};
+  int64_t wstats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // cached energies of the current state A
   bool stats_valid = false;
   double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
@@ -209,13 +209,16 @@ struct Chain : public ChainBase {
   int64_t nsum = 0;
   // stencil + boundary W (see build_stencil)
   struct WStencil {
-    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_dl, vg, vub, wcls, run_start, run_dl0, run_len, run_wo;
+    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_ldo, tiles;
     DevBuf<T> w, sub_w, Vl;
     DevBuf<int> brow, brp, bbase, banchor, tg, trp;
     DevBuf<uint16_t> bd, td;
     DevBuf<T> bw, tw, Vb;
     int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0;
-    long nvpad = 0;
+    // interior W^T v tiles: voxel tile dims, LDS box dims, lattice c + sg Z^3,
+    // smallest lattice offset, lattice grid box (lo, lv)
+    int n_tiles = 0, ncls = 1, tdim[3] = {1, 1, 1}, ldim[3] = {1, 1, 1}, c[3] = {0, 0, 0}, sg[3] = {1, 1, 1},
+        omin[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, lv[3] = {1, 1, 1};
   } ws;
   long pstride = 0;
   int slot_nblk[NSLOT] = {0};
@@ -765,7 +768,7 @@ struct Chain : public ChainBase {
     auto flatL = [&](const long* u) { return (u[0] - lo[0]) + L[0] * ((u[1] - lo[1]) + L[1] * (u[2] - lo[2])); };
     if ((long)nD * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of k_wmu_int
     const int ncls = (int)(sg[0] * sg[1] * sg[2]);
-    std::vector<int32_t> sub_start(ncls + 1, 0), sub_dl(nD);
+    std::vector<int32_t> sub_start(ncls + 1, 0), sub_o(3 * (size_t)nD);
     std::vector<T> sub_w(nD);
     {
       std::vector<int> scls(nD);
@@ -781,7 +784,7 @@ struct Chain : public ChainBase {
         long o[3];
         for (int a = 0; a < 3; ++a) o[a] = -fdiv(d3[3 * s + a], sg[a]);
         const int k = fill[scls[s]]++;
-        sub_dl[k] = (int32_t)(o[0] + L[0] * (o[1] + L[1] * o[2]));
+        for (int a = 0; a < 3; ++a) sub_o[3 * k + a] = (int32_t)o[a];
         sub_w[k] = wD[s];
       }
     }
@@ -792,67 +795,41 @@ struct Chain : public ChainBase {
       for (int a = 0; a < 3; ++a) u[a] = (b[a] - c[a]) / sg[a];
       ilidx[k] = (int32_t)flatL(u);
     }
-    // 3. data voxels bucketed by residue class; within a bucket, up to 4
-    //    voxels at consecutive lattice indices form one lane's group (their
-    //    reads of a sub-stencil x-run overlap); groups padded to wavefronts.
-    //    The sub-stencil of each class is cut into x-runs of at most 8 entries
-    //    (consecutive entries whose lattice offset falls by one).
-    std::vector<std::vector<int32_t>> bg(ncls), bu(ncls);
-    long safe_ub = -1;
+    // 3. interior W^T v by tiles: voxel tiles T0 x T1 x T2 that hold a data
+    //    voxel; each stages the lattice box its voxels reach (LT0 x LT1 x LT2)
+    //    in LDS.  T = (32, 4, 2) lattice steps per axis, so one residue class
+    //    of a tile is 256 voxels with 32 consecutive lattice x per half-wave;
+    //    halved (z, then y, then x) until the box fits 48 KiB of LDS.
     std::vector<int32_t> gidx(n_h);
-    for (long d = 0; d < n_h; ++d) {
-      gidx[d] = (int32_t)region_index(ijk[3 * d], ijk[3 * d + 1], ijk[3 * d + 2]);
-      long x[3], u[3], rho[3];
-      xyz(gidx[d], x);
-      for (int a = 0; a < 3; ++a) {
-        rho[a] = fmod_(x[a] - c[a], sg[a]);
-        u[a] = fdiv(x[a] - c[a], sg[a]);
-      }
-      const int k = (int)(rho[0] + sg[0] * (rho[1] + sg[1] * rho[2]));
-      const long f = flatL(u);
-      if (safe_ub < 0) safe_ub = f;
-      bg[k].push_back(gidx[d]);
-      bu[k].push_back((int32_t)f);
-    }
-    if (safe_ub < 0) safe_ub = 0;
-    std::vector<int32_t> gub, gg, wcls;
-    auto push_group = [&](int32_t ub, const int32_t* g4) {
-      gub.push_back(ub);
-      gg.insert(gg.end(), g4, g4 + 4);
+    for (long d = 0; d < n_h; ++d) gidx[d] = (int32_t)region_index(ijk[3 * d], ijk[3 * d + 1], ijk[3 * d + 2]);
+    long Tt[3] = {32 * sg[0], 4 * sg[1], 2 * sg[2]}, LT[3];
+    auto box_dims = [&](const long* t, long* lt) {
+      for (int a = 0; a < 3; ++a) lt[a] = (t[a] + sg[a] - 1) / sg[a] + 1 + (omax[a] - omin[a]);
     };
-    const int32_t none4[4] = {-1, -1, -1, -1};
-    for (int k = 0; k < ncls; ++k) {
-      if (bg[k].empty()) continue;
-      const size_t n = bg[k].size(), g0 = gub.size();
-      for (size_t i = 0; i < n;) {
-        int32_t g4[4] = {bg[k][i], -1, -1, -1};
-        int cnt = 1;
-        while (cnt < 4 && i + cnt < n && bu[k][i + cnt] == bu[k][i] + cnt) {
-          g4[cnt] = bg[k][i + cnt];
-          ++cnt;
-        }
-        push_group(bu[k][i], g4);
-        i += cnt;
+    while (true) {
+      box_dims(Tt, LT);
+      if (LT[0] * LT[1] * LT[2] * (long)sizeof(T) <= 48 * 1024) break;
+      const int ax = Tt[2] > sg[2] ? 2 : Tt[1] > sg[1] ? 1 : Tt[0] > sg[0] ? 0 : -1;
+      if (ax < 0) return false;  // stencil too wide for one LDS box
+      Tt[ax] = std::max(sg[ax], Tt[ax] / 2);
+    }
+    const long nT[3] = {cdiv(reg[0], Tt[0]), cdiv(reg[1], Tt[1]), cdiv(reg[2], Tt[2])};
+    std::vector<char> has(nT[0] * nT[1] * nT[2], 0);
+    for (long d = 0; d < n_h; ++d) {
+      long x[3];
+      xyz(gidx[d], x);
+      has[x[0] / Tt[0] + nT[0] * (x[1] / Tt[1] + nT[1] * (x[2] / Tt[2]))] = 1;
+    }
+    std::vector<int32_t> tiles;
+    for (long t = 0; t < (long)has.size(); ++t)
+      if (has[t]) {
+        tiles.push_back((int32_t)((t % nT[0]) * Tt[0]));
+        tiles.push_back((int32_t)(((t / nT[0]) % nT[1]) * Tt[1]));
+        tiles.push_back((int32_t)((t / (nT[0] * nT[1])) * Tt[2]));
       }
-      while ((gub.size() - g0) % 64) push_group((int32_t)safe_ub, none4);
-      for (size_t w = g0; w < gub.size(); w += 64) wcls.push_back(k);
-    }
-    while (gub.size() % kThreads || gub.empty()) {
-      push_group((int32_t)safe_ub, none4);
-      if (gub.size() % 64 == 1) wcls.push_back(0);
-    }
-    std::vector<int32_t> run_start(ncls + 1, 0), run_dl0, run_len, run_wo;
-    for (int k = 0; k < ncls; ++k) {
-      for (int t = sub_start[k]; t < sub_start[k + 1];) {
-        int m = 1;
-        while (m < 8 && t + m < sub_start[k + 1] && sub_dl[t + m] == sub_dl[t] - m) ++m;
-        run_dl0.push_back(sub_dl[t]);
-        run_len.push_back(m);
-        run_wo.push_back(t);
-        t += m;
-      }
-      run_start[k + 1] = (int32_t)run_dl0.size();
-    }
+    std::vector<int32_t> sub_ldo(nD);
+    for (int e = 0; e < nD; ++e)
+      sub_ldo[e] = (int32_t)(sub_o[3 * e] + LT[0] * (sub_o[3 * e + 1] + LT[1] * sub_o[3 * e + 2]));
     // 4. boundary rows: packed offsets from the row's bounding-box corner and
     //    a distinct anchor position (one of the row's own entries) for v
     std::vector<int32_t> brow, brp{0}, bbase, banchor;
@@ -950,26 +927,42 @@ struct Chain : public ChainBase {
         DR_REQUIRE((long)ibase[k] + fmin >= 0 && (long)ibase[k] + fmax < nreg, DR_ELOGIC,
                    "W stencil: interior entry outside the region");
       }
-      std::vector<long> smin(ncls, LONG_MAX), smax(ncls, LONG_MIN);
-      for (int k = 0; k < ncls; ++k)
-        for (int t = sub_start[k]; t < sub_start[k + 1]; ++t) {
-          smin[k] = std::min(smin[k], (long)sub_dl[t]);
-          smax[k] = std::max(smax[k], (long)sub_dl[t]);
+      // interior W^T v tiles: every voxel of every class reads LDS cells
+      // base + ldo with base = its lattice point less the box origin
+      const long ncell = LT[0] * LT[1] * LT[2];
+      for (size_t t = 0; t < tiles.size(); t += 3) {
+        long X0[3], u0[3];
+        for (int a = 0; a < 3; ++a) {
+          X0[a] = tiles[t + a];
+          DR_REQUIRE(X0[a] >= 0 && X0[a] < reg[a], DR_ELOGIC, "W stencil: tile outside the region");
+          u0[a] = fdiv(X0[a] - c[a], sg[a]) + omin[a];
         }
-      DR_REQUIRE(wcls.size() * 64 == gub.size() && gub.size() % kThreads == 0, DR_ELOGIC,
-                 "W stencil: voxel group padding");
-      // a group's window for a run reads lattice indices ub + dl0 + 3 - q,
-      // q in [0, len + 2]: within [ub + dl_min, ub + dl_max + 3]
-      for (size_t t = 0; t < gub.size(); ++t) {
-        const int k = wcls[t / 64];
-        DR_REQUIRE(k >= 0 && k < ncls, DR_ELOGIC, "W stencil: residue class out of range");
-        if (sub_start[k + 1] > sub_start[k])
-          DR_REQUIRE(gub[t] + smin[k] >= 0 && gub[t] + smax[k] + 3 < nVl + 4, DR_ELOGIC,
-                     "W stencil: sub-stencil read outside the lattice grid");
-        for (int p = 0; p < 4; ++p) DR_REQUIRE(gg[4 * t + p] < nreg, DR_ELOGIC, "W stencil: voxel outside the region");
+        for (int k = 0; k < ncls; ++k) {
+          const long rho[3] = {k % sg[0], (k / sg[0]) % sg[1], k / (sg[0] * sg[1])};
+          long bmin = 0, bmax = 0, st = 1;
+          bool empty = false;
+          for (int a = 0; a < 3; ++a) {
+            const long r = fmod_(rho[a] - (X0[a] - c[a]), sg[a]);
+            if (r >= Tt[a]) empty = true;
+            const long n = r < Tt[a] ? (Tt[a] - r + sg[a] - 1) / sg[a] : 0;
+            const long ub = fdiv(X0[a] + r - c[a], sg[a]) - u0[a];
+            // per axis: ub + j + o in [0, LT) for j < n and o in [omin, omax]
+            DR_REQUIRE(ub + omin[a] >= 0 && ub + n - 1 + omax[a] < LT[a], DR_ELOGIC,
+                       "W stencil: tile voxel reach outside its box");
+            bmin += st * ub;
+            bmax += st * (ub + n - 1);
+            st *= LT[a];
+          }
+          if (empty) continue;
+          for (int e = sub_start[k]; e < sub_start[k + 1]; ++e)
+            DR_REQUIRE(bmin + sub_ldo[e] >= 0 && bmax + sub_ldo[e] < ncell, DR_ELOGIC,
+                       "W stencil: sub-stencil read outside the tile box");
+        }
       }
-      for (size_t r = 0; r < run_dl0.size(); ++r)
-        DR_REQUIRE(run_len[r] >= 1 && run_len[r] <= 8, DR_ELOGIC, "W stencil: sub-stencil run length");
+      for (int e = 0; e < nD; ++e)
+        for (int a = 0; a < 3; ++a)
+          DR_REQUIRE(sub_o[3 * e + a] >= omin[a] && sub_o[3 * e + a] <= omax[a], DR_ELOGIC,
+                     "W stencil: sub-stencil offset outside the box margin");
     }
     for (size_t k = 0; k < brow.size(); ++k) {
       DR_REQUIRE(banchor[k] < nreg, DR_ELOGIC, "W stencil: anchor outside the region");
@@ -990,22 +983,26 @@ struct Chain : public ChainBase {
     ws.n_bnd = (int)brow.size();
     ws.n_touch = (int)tg.size();
     ws.dc = (int)(15 + 15 * R0 + 15 * R01);
-    ws.nvpad = (long)gub.size();
+    ws.n_tiles = (int)(tiles.size() / 3);
+    for (int a = 0; a < 3; ++a) {
+      ws.tdim[a] = (int)Tt[a];
+      ws.ldim[a] = (int)LT[a];
+      ws.c[a] = (int)c[a];
+      ws.sg[a] = (int)sg[a];
+      ws.omin[a] = (int)omin[a];
+      ws.lo[a] = (int)lo[a];
+      ws.lv[a] = (int)L[a];
+    }
+    ws.ncls = ncls;
     upload(ws.irow, irow);
     upload(ws.ibase, ibase);
     upload(ws.ilidx, ilidx);
     upload(ws.doff, D);
     upload(ws.w, wD);
     upload(ws.sub_start, sub_start);
-    upload(ws.sub_dl, sub_dl);
+    upload(ws.sub_ldo, sub_ldo);
     upload(ws.sub_w, sub_w);
-    upload(ws.vg, gg);
-    upload(ws.vub, gub);
-    upload(ws.wcls, wcls);
-    upload(ws.run_start, run_start);
-    upload(ws.run_dl0, run_dl0);
-    upload(ws.run_len, run_len);
-    upload(ws.run_wo, run_wo);
+    upload(ws.tiles, tiles);
     upload(ws.brow, brow);
     upload(ws.brp, brp);
     upload(ws.bbase, bbase);
@@ -1016,7 +1013,7 @@ struct Chain : public ChainBase {
     upload(ws.trp, trp);
     upload(ws.td, tdall);
     upload(ws.tw, twall);
-    ws.Vl.alloc(nVl + 4);  // + the reads of a group's absent members past the end
+    ws.Vl.alloc(std::max<long>(nVl, 1));
     ws.Vl.zero(stream);
     ws.Vb.alloc(std::max<long>(nreg, 1));
     ws.Vb.zero(stream);
@@ -1027,7 +1024,9 @@ struct Chain : public ChainBase {
     wstats[3] = nb_real;
     wstats[4] = nnz;
     wstats[5] = (int64_t)tg.size();
-    wstats[6] = (int64_t)gub.size();
+    wstats[6] = (int64_t)(tiles.size() / 3);
+    wstats[7] = LT[0] * LT[1] * LT[2];
+    wstats[8] = Tt[0] * Tt[1] * Tt[2];
     return true;
   }
 
@@ -1148,11 +1147,29 @@ struct Chain : public ChainBase {
         DR_HIP(hipGetLastError());
       }
       mark();
-      const int ti = (int)(ws.nvpad / kThreads), tb = cdiv(ws.n_touch, kThreads / kWtvLanes);
-      const WtvInt<T> wi{ws.vg.p, ws.vub.p, ws.wcls.p, ws.run_start.p, ws.run_dl0.p, ws.run_len.p, ws.run_wo.p,
-                         ws.sub_w.p, ws.Vl.p, WTv.p};
+      const int ti = ws.n_tiles, tb = cdiv(ws.n_touch, kThreads / kWtvLanes);
+      WtvInt<T> wi{};
+      wi.tiles = ws.tiles.p;
+      wi.cls_start = ws.sub_start.p;
+      wi.ldo = ws.sub_ldo.p;
+      wi.w = reinterpret_cast<const typename Bits<T>::type*>(ws.sub_w.p);
+      wi.Vl = ws.Vl.p;
+      wi.mask = mask.p;
+      wi.out = WTv.p;
+      wi.ncls = ws.ncls;
+      for (int a = 0; a < 3; ++a) {
+        wi.TD[a] = ws.tdim[a];
+        wi.LT[a] = ws.ldim[a];
+        wi.c[a] = ws.c[a];
+        wi.sg[a] = ws.sg[a];
+        wi.omin[a] = ws.omin[a];
+        wi.lo[a] = ws.lo[a];
+        wi.lv[a] = ws.lv[a];
+        wi.R[a] = reg[a];
+      }
       const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p};
-      hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), 0, stream, wi, wb, ti);
+      const size_t lds_t = (size_t)ws.ldim[0] * ws.ldim[1] * ws.ldim[2] * sizeof(T);
+      hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), lds_t, stream, wi, wb, ti);
       DR_HIP(hipGetLastError());
       mark();
       return;
@@ -1647,7 +1664,7 @@ int dr_w_stats(dr_chain* c, int64_t* stats) {
   return guarded([&] {
     auto& k = *c->impl;
     stats[0] = k.use_cls ? 1 : 0;
-    for (int i = 0; i < 7; ++i) stats[1 + i] = k.wstats[i];
+    for (int i = 0; i < 9; ++i) stats[1 + i] = k.wstats[i];
   });
 }
 

@@ -392,9 +392,18 @@ template <typename T> struct WmuBnd {
   int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
   int R0, R01; T* Vb;
 };
+// bit patterns of T: weights read through them are wave-uniform loads the
+// compiler may serve from the scalar cache (a T pointer could alias the output)
+template <typename T> struct Bits;
+template <> struct Bits<float> { using type = uint32_t; };
+template <> struct Bits<double> { using type = uint64_t; };
+__device__ __forceinline__ float from_bits(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ double from_bits(uint64_t u) { return __longlong_as_double((long long)u); }
+
 template <typename T> struct WtvInt {
-  const int* g4; const int* ub; const int* wcls; const int* run_start; const int* run_dl0; const int* run_len;
-  const int* run_wo; const T* sub_w; const T* Vl; T* out;
+  const int* tiles; const int* cls_start; const int* ldo; const typename Bits<T>::type* w; const T* Vl;
+  const uint8_t* mask; T* out;
+  int ncls, TD[3], LT[3], c[3], sg[3], omin[3], lo[3], lv[3], R[3];
 };
 template <typename T> struct WtvBnd {
   int n; const int* g; const int* rp; const uint16_t* d; const T* w; int R0, R01, dc; const T* Vb; T* out;
@@ -490,41 +499,101 @@ k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, int gi, const T* __restrict__ R, const T* _
   if (part) red_write(e2, red, part, blockIdx.x);
 }
 
-// W^T v of the interior rows at every data voxel (written, not added).  The
-// voxels are bucketed by residue class, so a wavefront reads one class's
-// sub-stencil with scalar loads.  A lane takes a group of up to 4 voxels at
-// consecutive lattice indices ub .. ub+3: for an x-run of m entries (lattice
-// offsets dl0, dl0 - 1, ..., dl0 - m + 1) the four voxels read the window
-// Vl[ub + dl0 + 3 - q], q = 0 .. m + 2, once (m + 3 gathers instead of 4 m),
-// entry k of voxel p being x[3 - p + k].  Each voxel sums its entries in
-// sub-stencil order.
+// W^T v of the interior rows (written, not added), one voxel tile per
+// workgroup.  The tile T0 x T1 x T2 stages the box of lattice values v (Vl,
+// zero outside the lattice grid) that its voxels reach in LDS, then walks the
+// residue classes: a voxel x of class rho (x = c + sg u + rho) takes
+// sum_e w_e v[u + o_e] over its class's sub-stencil, in sub-stencil order, with
+// the entry offsets o_e pre-flattened into the box (ldo).  Lanes run along
+// lattice x, so the LDS reads of a wavefront are consecutive.  Data voxels get
+// the sum, other region voxels of the tile 0.  Weights are passed as bit
+// patterns (Bits<T>).
+__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
 template <typename T>
 __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
-  const int t = blk * kThreads + threadIdx.x;
-  const int cls = __builtin_amdgcn_readfirstlane(a.wcls[t >> 6]);
-  const int r0 = a.run_start[cls], r1 = a.run_start[cls + 1];
-  const T* V = a.Vl + a.ub[t];
-  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
-  for (int r = r0; r < r1; ++r) {
-    const int dl0 = a.run_dl0[r], m = a.run_len[r], wo = a.run_wo[r];
-    T x[11];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* sv = reinterpret_cast<T*>(lds_raw);
+  int X0[3], u0[3];
 #pragma unroll
-    for (int q = 0; q < 11; ++q) x[q] = q < m + 3 ? V[dl0 + 3 - q] : (T)0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k < m) {
-        const T w = a.sub_w[wo + k];
-        acc[0] += w * x[3 + k];
-        acc[1] += w * x[2 + k];
-        acc[2] += w * x[1 + k];
-        acc[3] += w * x[k];
-      }
+  for (int d = 0; d < 3; ++d) {
+    X0[d] = a.tiles[3 * blk + d];
+    u0[d] = floordiv(X0[d] - a.c[d], a.sg[d]) + a.omin[d];
+  }
+  // stage the box: lattice rows (i1, i2) round-robin over the wavefronts
+  for (int row = threadIdx.x >> 6; row < a.LT[1] * a.LT[2]; row += kThreads / 64) {
+    const int i1 = row % a.LT[1], i2 = row / a.LT[1];
+    const int v1 = u0[1] + i1 - a.lo[1], v2 = u0[2] + i2 - a.lo[2];
+    const bool yz = v1 >= 0 && v1 < a.lv[1] && v2 >= 0 && v2 < a.lv[2];
+    for (int i0 = threadIdx.x & 63; i0 < a.LT[0]; i0 += 64) {
+      const int v0 = u0[0] + i0 - a.lo[0];
+      T v = (T)0;
+      if (yz && v0 >= 0 && v0 < a.lv[0]) v = a.Vl[v0 + (long)a.lv[0] * (v1 + (long)a.lv[1] * v2)];
+      sv[i0 + a.LT[0] * row] = v;
     }
   }
+  __syncthreads();
+  // one residue class per wavefront, four voxels per lane: each batch of 8
+  // (offset, weight) pairs feeds 32 independent LDS reads and multiply-adds
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  for (int k = wave; k < a.ncls; k += kThreads / 64) {
+    const int rho[3] = {k % a.sg[0], (k / a.sg[0]) % a.sg[1], k / (a.sg[0] * a.sg[1])};
+    int r[3], n[3], ub[3];
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const int gj = a.g4[4 * t + p];
-    if (gj >= 0) a.out[gj] = acc[p];
+    for (int d = 0; d < 3; ++d) {
+      r[d] = ((rho[d] - (X0[d] - a.c[d])) % a.sg[d] + a.sg[d]) % a.sg[d];
+      n[d] = r[d] < a.TD[d] ? (a.TD[d] - r[d] + a.sg[d] - 1) / a.sg[d] : 0;
+      ub[d] = floordiv(X0[d] + r[d] - a.c[d], a.sg[d]) - u0[d];
+    }
+    const int nv = n[0] * n[1] * n[2];
+    const int e0 = a.cls_start[k], e1 = a.cls_start[k + 1];
+    for (int f0 = 0; f0 < nv; f0 += 4 * 64) {  // uniform bounds: entry loads stay scalar
+      int bs[4], jj[4][3];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int f = f0 + lane + 64 * p < nv ? f0 + lane + 64 * p : 0;
+        jj[p][0] = f % n[0];
+        jj[p][1] = (f / n[0]) % n[1];
+        jj[p][2] = f / (n[0] * n[1]);
+        bs[p] = (ub[0] + jj[p][0]) + a.LT[0] * ((ub[1] + jj[p][1]) + a.LT[1] * (ub[2] + jj[p][2]));
+      }
+      T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+      int e = e0;
+      for (; e + 8 <= e1; e += 8) {
+        int o[8];
+        T w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          o[q] = a.ldo[e + q];
+          w[q] = from_bits(a.w[e + q]);
+        }
+        T v[4][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int p = 0; p < 4; ++p) v[p][q] = sv[bs[p] + o[q]];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int p = 0; p < 4; ++p) acc[p] += w[q] * v[p][q];
+      }
+      for (; e < e1; ++e) {
+        const int o = a.ldo[e];
+        const T w = from_bits(a.w[e]);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) acc[p] += w * sv[bs[p] + o];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        if (f0 + lane + 64 * p >= nv) continue;
+        const int x0 = X0[0] + r[0] + a.sg[0] * jj[p][0], x1 = X0[1] + r[1] + a.sg[1] * jj[p][1],
+                  x2 = X0[2] + r[2] + a.sg[2] * jj[p][2];
+        if (x0 < a.R[0] && x1 < a.R[1] && x2 < a.R[2]) {
+          const long g = x0 + (long)a.R[0] * (x1 + (long)a.R[1] * x2);
+          a.out[g] = a.mask[g] ? acc[p] : (T)0;
+        }
+      }
+    }
   }
 }
 

@@ -117,8 +117,9 @@ int dr_set_data(dr_chain* c, const double* Yh, int64_t n_h, const double* Ys, in
 int dr_w_layout(dr_chain* c, int force_csr);
 /* stats[0] = stencil layout active; [1] interior (stencil) rows; [2] stencil
  * entries; [3] boundary rows; [4] boundary entries; [5] nnz(W); [6] data
- * voxels touched by boundary rows; [7] voxel groups (up to 4 lattice-neighbour
- * voxels each, padded to wavefronts) of the interior W^T v pass. */
+ * voxels touched by boundary rows; [7] voxel tiles of the interior W^T v pass
+ * (tiles holding a data voxel); [8] lattice cells staged per tile; [9] voxels
+ * per tile.  stats must hold 10 entries. */
 int dr_w_stats(dr_chain* c, int64_t* stats);
 
 /* mcmc_summary<T> (gaussian_process_model.h:191-224) kept on the device:

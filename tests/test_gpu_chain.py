@@ -214,6 +214,7 @@ def test_w_stencil_layout_matches_csr(gpu_available, precision, dims):
             assert st["interior_rows"] > 0 and st["boundary_rows"] > 0
             assert st["interior_rows"] + st["boundary_rows"] == p["W"].shape[0]
             assert st["stencil_entries"] * st["interior_rows"] + st["boundary_entries"] == p["W"].nnz
+            assert st["interior_tiles"] > 0 and st["tile_cells"] * (4 if precision == "fp32" else 8) <= 48 * 1024
         rng = np.random.default_rng(5)
         z = (rng.standard_normal(c.N), rng.standard_normal(c.N))
         outs.append((c.gradient(), c.update(0.05, 3, 0.5, noise=z), c.log_likelihood()))
