@@ -385,13 +385,6 @@ __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 constexpr int kWtvLanes = 8;
 
 // Argument bundles of the four parts of the two products.
-template <typename T> struct WmuInt {
-  int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
-};
-template <typename T> struct WmuBnd {
-  int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
-  int R0, R01; T* Vb;
-};
 // bit patterns of T: weights read through them are wave-uniform loads the
 // compiler may serve from the scalar cache (a T pointer could alias the output)
 template <typename T> struct Bits;
@@ -400,6 +393,13 @@ template <> struct Bits<double> { using type = uint64_t; };
 __device__ __forceinline__ float from_bits(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ double from_bits(uint64_t u) { return __longlong_as_double((long long)u); }
 
+template <typename T> struct WmuInt {
+  int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
+};
+template <typename T> struct WmuBnd {
+  int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
+  int R0, R01; T* Vb;
+};
 template <typename T> struct WtvInt {
   const int* tiles; const int* cls_start; const int* ldo; const typename Bits<T>::type* w; const T* Vl;
   const uint8_t* mask; T* out;
@@ -520,16 +520,27 @@ __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
     X0[d] = a.tiles[3 * blk + d];
     u0[d] = floordiv(X0[d] - a.c[d], a.sg[d]) + a.omin[d];
   }
-  // stage the box: lattice rows (i1, i2) round-robin over the wavefronts
-  for (int row = threadIdx.x >> 6; row < a.LT[1] * a.LT[2]; row += kThreads / 64) {
-    const int i1 = row % a.LT[1], i2 = row / a.LT[1];
-    const int v1 = u0[1] + i1 - a.lo[1], v2 = u0[2] + i2 - a.lo[2];
-    const bool yz = v1 >= 0 && v1 < a.lv[1] && v2 >= 0 && v2 < a.lv[2];
-    for (int i0 = threadIdx.x & 63; i0 < a.LT[0]; i0 += 64) {
+  // stage the box: 8 lattice rows (i1, i2) per wavefront per trip, 8 loads in
+  // flight per lane
+  {
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    const int nrow = a.LT[1] * a.LT[2];
+    for (int i0 = ln; i0 < a.LT[0]; i0 += 64) {
       const int v0 = u0[0] + i0 - a.lo[0];
-      T v = (T)0;
-      if (yz && v0 >= 0 && v0 < a.lv[0]) v = a.Vl[v0 + (long)a.lv[0] * (v1 + (long)a.lv[1] * v2)];
-      sv[i0 + a.LT[0] * row] = v;
+      const bool xin = v0 >= 0 && v0 < a.lv[0];
+      for (int r0 = wv * 8; r0 < nrow; r0 += 8 * (kThreads / 64)) {
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int row = r0 + q, v1 = u0[1] + row % a.LT[1] - a.lo[1], v2 = u0[2] + row / a.LT[1] - a.lo[2];
+          v[q] = (T)0;
+          if (row < nrow && xin && v1 >= 0 && v1 < a.lv[1] && v2 >= 0 && v2 < a.lv[2])
+            v[q] = a.Vl[v0 + (long)a.lv[0] * (v1 + (long)a.lv[1] * v2)];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (r0 + q < nrow) sv[i0 + a.LT[0] * (r0 + q)] = v[q];
+      }
     }
   }
   __syncthreads();
