@@ -640,11 +640,12 @@ struct Chain : public ChainBase {
     if (!h.empty()) DR_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, stream));
   }
 
-  // Stencil + boundary layout of W (kernels k_wmu_int / k_wmu_bnd / k_wtv_int /
-  // k_wtv_bnd).  cols: image-region grid index of every entry.  Returns false
-  // (plain CSR kernels) when a boundary row spans more than 16 voxels on an
-  // axis or has no free anchor position.
-  // Entry order for the quad loads of k_wmu_bnd / k_wtv_bnd: a list
+  // Stencil + boundary layout of W (interior and boundary blocks of k_wmu_sb
+  // and k_wtv_sb).  cols: image-region grid index of every entry.  Returns
+  // false (plain CSR kernels) when a boundary row spans more than 16 voxels on
+  // an axis, has no free anchor position, or the stencil is too wide for the
+  // LDS table of W mu or the lattice box of a W^T v tile.
+  // Entry order for the quad loads of the boundary blocks: a list
   // [start, end) (a multiple of 4) is cut into blocks of `lanes` quads; in a
   // block of nq quads, quad l holds entries l, l + nq, l + 2 nq, l + 3 nq, so
   // the k-th gather of all lanes reads consecutive entries.
@@ -766,7 +767,7 @@ struct Chain : public ChainBase {
     const long nVl = L[0] * L[1] * L[2];
     DR_REQUIRE(nVl < (1L << 31), DR_EDOMAIN, "W stencil lattice too large");
     auto flatL = [&](const long* u) { return (u[0] - lo[0]) + L[0] * ((u[1] - lo[1]) + L[1] * (u[2] - lo[2])); };
-    if ((long)nD * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of k_wmu_int
+    if ((long)nD * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of the interior W mu blocks
     const int ncls = (int)(sg[0] * sg[1] * sg[2]);
     std::vector<int32_t> sub_start(ncls + 1, 0), sub_o(3 * (size_t)nD);
     std::vector<T> sub_w(nD);
