@@ -141,7 +141,7 @@ struct ChainBase {
   bool data_set = false;
   bool use_cls = false;    // class-stencil W kernels active
   bool force_csr = false;  // keep the plain CSR kernels (tests)
-  int64_t wstats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t wstats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // cached energies of the current state A
   bool stats_valid = false;
   double prior_sum = 0.0, sse_h = 0.0, sse_s = 0.0;
@@ -211,10 +211,10 @@ struct Chain : public ChainBase {
   struct WStencil {
     DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_ldo, tiles;
     DevBuf<T> w, sub_w, Vl;
-    DevBuf<int> brow, brp, bbase, banchor, tg, trp;
+    DevBuf<int> brow, brp, bbase, banchor, tg, trp, frow, frp, fcol, tfp, tfidx;
     DevBuf<uint16_t> bd, td;
-    DevBuf<T> bw, tw, Vb;
-    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0;
+    DevBuf<T> bw, tw, Vb, fw, tfw, Vf;
+    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0, n_far = 0;
     // interior W^T v tiles: voxel tile dims, LDS box dims, lattice c + sg Z^3,
     // smallest lattice offset, lattice grid box (lo, lv)
     int n_tiles = 0, ncls = 1, tdim[3] = {1, 1, 1}, ldim[3] = {1, 1, 1}, c[3] = {0, 0, 0}, sg[3] = {1, 1, 1},
@@ -832,12 +832,17 @@ struct Chain : public ChainBase {
     for (int e = 0; e < nD; ++e)
       sub_ldo[e] = (int32_t)(sub_o[3 * e] + LT[0] * (sub_o[3 * e + 1] + LT[1] * sub_o[3 * e + 2]));
     // 4. boundary rows: packed offsets from the row's bounding-box corner and
-    //    a distinct anchor position (one of the row's own entries) for v
+    //    a distinct anchor position (one of the row's own entries) for v.
+    //    Rows that span more than 16 voxels on an axis (the reference's
+    //    flat-offset neighbourhood can wrap across a row end) or find no free
+    //    anchor are far rows: explicit region indices, v in a compact array.
     std::vector<int32_t> brow, brp{0}, bbase, banchor;
     std::vector<uint16_t> bd;
     std::vector<T> bw;
     std::vector<uint8_t> claimed(nreg, 0);
-    std::vector<int32_t> row_anchor(n_s, -1);
+    std::vector<int32_t> row_anchor(n_s, -1), row_far(n_s, -1);
+    std::vector<int32_t> frow, frp{0}, fcol;
+    std::vector<T> fwt;
     long nb_real = 0;
     for (long i = 0; i < n_s; ++i) {
       if (interior[i]) continue;
@@ -851,14 +856,25 @@ struct Chain : public ChainBase {
       }
       int32_t an = -1;
       if (rp[i + 1] > rp[i]) {
-        for (int a = 0; a < 3; ++a)
-          if (mx[a] - mn[a] > 15) return false;
-        for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
-          if (!claimed[cols[t]]) {
-            an = cols[t];
-            break;
+        bool wide = false;
+        for (int a = 0; a < 3; ++a) wide = wide || mx[a] - mn[a] > 15;
+        if (!wide)
+          for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
+            if (!claimed[cols[t]]) {
+              an = cols[t];
+              break;
+            }
+        if (an < 0) {  // far row
+          row_far[i] = (int32_t)frow.size();
+          frow.push_back((int32_t)i);
+          for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
+            fcol.push_back(cols[t]);
+            fwt.push_back(vals[t]);
           }
-        if (an < 0) return false;
+          frp.push_back((int32_t)fcol.size());
+          nb_real += rp[i + 1] - rp[i];
+          continue;
+        }
         claimed[an] = 1;
       }
       row_anchor[i] = an;
@@ -879,11 +895,12 @@ struct Chain : public ChainBase {
       banchor.push_back(an);
       brp.push_back((int32_t)bw.size());
     }
-    // 5. transposed boundary entries per touched data voxel, in row order
-    std::vector<int32_t> tcount(n_h + 1, 0);
+    // 5. transposed boundary entries per touched data voxel, in row order:
+    //    packed quads from the boundary rows, then (far row, weight) pairs
+    std::vector<int32_t> tcount(n_h + 1, 0), fcount(n_h + 1, 0);
     for (long i = 0; i < n_s; ++i)
       if (!interior[i])
-        for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) tcount[W_cols[t] + 1]++;
+        for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) (row_far[i] >= 0 ? fcount : tcount)[W_cols[t] + 1]++;
     std::vector<int32_t> tstart(n_h + 1, 0);
     for (long d = 0; d < n_h; ++d) tstart[d + 1] = tstart[d] + ((tcount[d + 1] + 3) & ~3);
     const long nbt = tstart[n_h];
@@ -893,7 +910,7 @@ struct Chain : public ChainBase {
     {
       std::vector<int32_t> fill(tstart.begin(), tstart.end() - 1);
       for (long i = 0; i < n_s; ++i) {
-        if (interior[i]) continue;
+        if (interior[i] || row_far[i] >= 0) continue;
         long xa[3], x[3];
         xyz(row_anchor[i] >= 0 ? row_anchor[i] : 0, xa);
         for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) {
@@ -902,7 +919,7 @@ struct Chain : public ChainBase {
           long q[3];
           for (int a = 0; a < 3; ++a) q[a] = x[a] - xa[a] + 15;
           for (int a = 0; a < 3; ++a)
-            if (q[a] < 0 || q[a] > 31) return false;
+            DR_REQUIRE(q[a] >= 0 && q[a] <= 31, DR_ELOGIC, "W stencil: transposed offset out of range");
           tdall[fill[dv]] = (uint16_t)(q[0] | (q[1] << 5) | (q[2] << 10));
           twall[fill[dv]] = vals[t];
           fill[dv]++;
@@ -910,12 +927,35 @@ struct Chain : public ChainBase {
       }
     }
     for (long d = 0; d < n_h; ++d) interleave_quads(tdall, twall, tstart[d], tstart[d + 1], kWtvLanes);
-    std::vector<int32_t> tg, trp{0};
+    std::vector<int32_t> fstart(n_h + 1, 0);
+    for (long d = 0; d < n_h; ++d) fstart[d + 1] = fstart[d] + fcount[d + 1];
+    std::vector<int32_t> tfidx(fstart[n_h]);
+    std::vector<T> tfw(fstart[n_h]);
+    {
+      std::vector<int32_t> fill(fstart.begin(), fstart.end() - 1);
+      for (long i = 0; i < n_s; ++i) {
+        if (row_far[i] < 0) continue;
+        for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) {
+          const int32_t dv = W_cols[t];
+          tfidx[fill[dv]] = row_far[i];
+          tfw[fill[dv]] = vals[t];
+          fill[dv]++;
+        }
+      }
+    }
+    std::vector<int32_t> tg, trp{0}, tfp{0}, tfp_start;
     for (long d = 0; d < n_h; ++d)
-      if (tstart[d + 1] > tstart[d]) {
+      if (tstart[d + 1] > tstart[d] || fstart[d + 1] > fstart[d]) {
         tg.push_back(gidx[d]);
         trp.push_back(tstart[d + 1]);
+        tfp_start.push_back(fstart[d]);
+        tfp.push_back(fstart[d + 1]);
       }
+    // packed ranges of touched voxels are [trp[k], trp[k + 1]) only when the
+    // untouched voxels between them have empty lists, which holds: tstart is
+    // cumulative over all data voxels and untouched voxels add nothing.
+    for (size_t k = 0; k < tfp_start.size(); ++k)
+      DR_REQUIRE(tfp_start[k] == tfp[k], DR_ELOGIC, "W stencil: far transposed ranges not contiguous");
     // 6. every device access of the four kernels stays inside its buffer
     {
       long fmin = 0, fmax = 0;
@@ -978,8 +1018,15 @@ struct Chain : public ChainBase {
                                      ((q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10));
         DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: transposed entry outside the region");
       }
+    for (size_t t = 0; t < fcol.size(); ++t)
+      DR_REQUIRE(fcol[t] >= 0 && fcol[t] < nreg, DR_ELOGIC, "W stencil: far-row entry outside the region");
+    for (size_t t = 0; t < tfidx.size(); ++t)
+      DR_REQUIRE(tfidx[t] >= 0 && tfidx[t] < (long)frow.size(), DR_ELOGIC, "W stencil: far row index out of range");
+    DR_REQUIRE(tfp.size() == tg.size() + 1 && trp.size() == tg.size() + 1, DR_ELOGIC,
+               "W stencil: transposed row pointers");
     // 7. upload
     ws.n_int = (int)n_int;
+    ws.n_far = (int)frow.size();
     ws.nD = nD;
     ws.n_bnd = (int)brow.size();
     ws.n_touch = (int)tg.size();
@@ -1014,6 +1061,15 @@ struct Chain : public ChainBase {
     upload(ws.trp, trp);
     upload(ws.td, tdall);
     upload(ws.tw, twall);
+    upload(ws.frow, frow);
+    upload(ws.frp, frp);
+    upload(ws.fcol, fcol);
+    upload(ws.fw, fwt);
+    upload(ws.tfp, tfp);
+    upload(ws.tfidx, tfidx);
+    upload(ws.tfw, tfw);
+    ws.Vf.alloc(std::max<long>((long)frow.size(), 1));
+    ws.Vf.zero(stream);
     ws.Vl.alloc(std::max<long>(nVl, 1));
     ws.Vl.zero(stream);
     ws.Vb.alloc(std::max<long>(nreg, 1));
@@ -1021,13 +1077,14 @@ struct Chain : public ChainBase {
     DR_HIP(hipStreamSynchronize(stream));
     wstats[0] = n_int;
     wstats[1] = nD;
-    wstats[2] = (int64_t)brow.size();
+    wstats[2] = (int64_t)(brow.size() + frow.size());
     wstats[3] = nb_real;
     wstats[4] = nnz;
     wstats[5] = (int64_t)tg.size();
     wstats[6] = (int64_t)(tiles.size() / 3);
     wstats[7] = LT[0] * LT[1] * LT[2];
     wstats[8] = Tt[0] * Tt[1] * Tt[2];
+    wstats[9] = (int64_t)frow.size();
     return true;
   }
 
@@ -1136,15 +1193,17 @@ struct Chain : public ChainBase {
     const int g1 = cdiv(n_s, kThreads / 64);
     const int g2 = cdiv(n_h, kThreads / 16);
     if (use_cls) {
-      const int gi = cdiv((long)ws.n_int * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64);
-      if (slot >= 0) note_blocks(slot, gi + gb);
+      const int gi = cdiv((long)ws.n_int * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
+                gf = cdiv(ws.n_far, kThreads / 64);
+      if (slot >= 0) note_blocks(slot, gi + gb + gf);
       const int R0 = reg[0], R01 = reg[0] * reg[1];
-      if (gi + gb > 0) {
+      if (gi + gb + gf > 0) {
         const WmuInt<T> ai{ws.n_int, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, ws.Vl.p};
         const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01,
                            ws.Vb.p};
-        hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream, ai,
-                           ab, gi, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
+        const WmuFar<T> af{ws.n_far, ws.frow.p, ws.frp.p, ws.fcol.p, ws.fw.p, ws.Vf.p};
+        hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb + gf), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream,
+                           ai, ab, af, gi, gb, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
         DR_HIP(hipGetLastError());
       }
       mark();
@@ -1168,7 +1227,8 @@ struct Chain : public ChainBase {
         wi.lv[a] = ws.lv[a];
         wi.R[a] = reg[a];
       }
-      const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p};
+      const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p,
+                         ws.tfp.p, ws.tfidx.p, ws.tfw.p, ws.Vf.p};
       const size_t lds_t = (size_t)ws.ldim[0] * ws.ldim[1] * ws.ldim[2] * sizeof(T);
       hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), lds_t, stream, wi, wb, ti);
       DR_HIP(hipGetLastError());
@@ -1665,7 +1725,7 @@ int dr_w_stats(dr_chain* c, int64_t* stats) {
   return guarded([&] {
     auto& k = *c->impl;
     stats[0] = k.use_cls ? 1 : 0;
-    for (int i = 0; i < 9; ++i) stats[1 + i] = k.wstats[i];
+    for (int i = 0; i < 10; ++i) stats[1 + i] = k.wstats[i];
   });
 }
 

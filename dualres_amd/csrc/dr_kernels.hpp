@@ -396,6 +396,10 @@ __device__ __forceinline__ double from_bits(uint64_t u) { return __longlong_as_d
 template <typename T> struct WmuInt {
   int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
 };
+// far rows: explicit region indices, v to the compact array Vf
+template <typename T> struct WmuFar {
+  int n; const int* row; const int* rp; const int* col; const T* w; T* Vf;
+};
 template <typename T> struct WmuBnd {
   int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
   int R0, R01; T* Vb;
@@ -407,6 +411,7 @@ template <typename T> struct WtvInt {
 };
 template <typename T> struct WtvBnd {
   int n; const int* g; const int* rp; const uint16_t* d; const T* w; int R0, R01, dc; const T* Vb; T* out;
+  const int* fp; const int* fidx; const T* fw; const T* Vf;  // far-row entries per voxel
 };
 
 // interior rows: 8 lanes per row, lane q takes stencil entries q, q+8, ...
@@ -486,16 +491,39 @@ __device__ __forceinline__ double wmu_bnd_block(const WmuBnd<T>& a, int blk, con
   return e2;
 }
 
+// far rows (neighbourhoods too wide to pack), one wavefront per row
+template <typename T>
+__device__ __forceinline__ double wmu_far_block(const WmuFar<T>& a, int blk, const T* __restrict__ R,
+                                                const T* __restrict__ Ys, T sig_s) {
+  const int lane = threadIdx.x & 63;
+  const int k = blk * (kThreads / 64) + (threadIdx.x >> 6);
+  double e2 = 0.0;
+  if (k < a.n) {
+    T acc = (T)0;
+    for (int j = a.rp[k] + lane; j < a.rp[k + 1]; j += 64) acc += a.w[j] * R[a.col[j]];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const T e = Ys[a.row[k]] - acc;
+      a.Vf[k] = sig_s * e;
+      e2 = (double)e * (double)e;
+    }
+  }
+  return e2;
+}
+
 // v = sigma_s (Ys - W R) for every row and ||Ys - W R||^2: blocks [0, gi)
-// take the interior rows, the rest the boundary rows, in one launch so the
-// latency-bound gathers and the streaming boundary entries overlap
+// take the interior rows, [gi, gi + gb) the packed boundary rows and the rest
+// the far rows, in one launch so the latency-bound gathers and the streaming
+// boundary entries overlap
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, int gi, const T* __restrict__ R, const T* __restrict__ Ys, T sig_s,
-         double* part) {
+k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, WmuFar<T> af, int gi, int gb, const T* __restrict__ R,
+         const T* __restrict__ Ys, T sig_s, double* part) {
   __shared__ double red[kThreads];
   const int blk = blockIdx.x;
-  const double e2 = blk < gi ? wmu_int_block(ai, blk, R, Ys, sig_s) : wmu_bnd_block(ab, blk - gi, R, Ys, sig_s);
+  const double e2 = blk < gi        ? wmu_int_block(ai, blk, R, Ys, sig_s)
+                    : blk < gi + gb ? wmu_bnd_block(ab, blk - gi, R, Ys, sig_s)
+                                    : wmu_far_block(af, blk - gi - gb, R, Ys, sig_s);
   if (part) red_write(e2, red, part, blockIdx.x);
 }
 
@@ -608,9 +636,10 @@ __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
   }
 }
 
-// W^T v of the boundary rows at the voxels they touch (written to its own
-// field, which the residual adds); 16 lanes per voxel, quads of entries as in
-// the W mu part (pad offset = the voxel itself, weight 0).  Entry offsets are
+// W^T v of the boundary and far rows at the voxels they touch (written to its
+// own field, which the residual adds); kWtvLanes lanes per voxel, quads of
+// entries as in the W mu part (pad offset = the voxel itself, weight 0), then
+// the voxel's far-row entries (far row index, weight).  Quad offsets are
 // voxel - anchor + 15 per axis, packed.
 template <typename T>
 __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
@@ -632,6 +661,7 @@ __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
       acc += wq.v[2] * Vg[-unpack3(lo16(dq.y), a.R0, a.R01)];
       acc += wq.v[3] * Vg[-unpack3(hi16(dq.y), a.R0, a.R01)];
     }
+    for (int j = a.fp[k] + lane; j < a.fp[k + 1]; j += kWtvLanes) acc += a.fw[j] * a.Vf[a.fidx[j]];
   }
 #pragma unroll
   for (int o = kWtvLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWtvLanes);

@@ -100,12 +100,12 @@ class Chain:
         L.check(self._lib.dr_w_layout(self._h, 1 if force_csr else 0))
 
     def w_stats(self):
-        st = np.zeros(10, dtype=np.int64)
+        st = np.zeros(11, dtype=np.int64)
         L.check(self._lib.dr_w_stats(self._h, L.iptr64(st)))
         return {"stencil_layout": bool(st[0]), "interior_rows": int(st[1]), "stencil_entries": int(st[2]),
                 "boundary_rows": int(st[3]), "boundary_entries": int(st[4]), "nnz": int(st[5]),
                 "touched_voxels": int(st[6]), "interior_tiles": int(st[7]), "tile_cells": int(st[8]),
-                "tile_voxels": int(st[9])}
+                "tile_voxels": int(st[9]), "far_rows": int(st[10])}
 
     def add_data_to_mu(self):
         L.check(self._lib.dr_add_data_to_mu(self._h))

@@ -119,7 +119,8 @@ int dr_w_layout(dr_chain* c, int force_csr);
  * entries; [3] boundary rows; [4] boundary entries; [5] nnz(W); [6] data
  * voxels touched by boundary rows; [7] voxel tiles of the interior W^T v pass
  * (tiles holding a data voxel); [8] lattice cells staged per tile; [9] voxels
- * per tile.  stats must hold 10 entries. */
+ * per tile; [10] far rows (boundary rows too wide to pack, counted in [3]).
+ * stats must hold 11 entries. */
 int dr_w_stats(dr_chain* c, int64_t* stats);
 
 /* mcmc_summary<T> (gaussian_process_model.h:191-224) kept on the device:

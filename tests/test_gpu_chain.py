@@ -331,3 +331,59 @@ def test_fit_with_tau_update(gpu_available):
     assert np.isfinite(t) and t > 0 and abs(t - 1 / THETA2[0]) > 1e-6
     assert np.all(np.isfinite(summ.mode_mu()))
     c.close()
+
+
+def _ratio_problem(dims, ratio, theta, radius, seed=3):
+    """Highres image on `dims` at 1 mm and a stdres image at `ratio` mm whose
+    voxel centres fall on highres voxels (bases on the lattice ratio * Z^3)."""
+    from dualres_amd import problem
+    from oracle import kriging as okr
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    m = problem.ellipsoid_mask(dims, 0.45)
+    hr = np.where(m, rng.standard_normal(m.shape) + 3.0, 0.0)
+    q_hr = problem.qform_axis(1.0, dims)
+    sd = tuple(max(2, d // ratio) for d in dims)
+    sm = problem.ellipsoid_mask(sd, 0.45)
+    std = np.where(sm, rng.standard_normal(sm.shape) + 3.0, 0.0)
+    off = (ratio - 1) / 2.0
+    q_std = problem.qform_axis(float(ratio), sd, origin=[q_hr[a, 3] + off for a in range(3)])
+    rp, cols, vals, nrow, ncol, _ = okr.get_sparse_kriging_matrix_data(hr, std, q_hr, q_std, theta, radius)
+    p = {"hr": hr, "q_hr": q_hr, "std": std, "q_std": q_std, "ijk": problem.nonzero_ijk(hr),
+         "Yh": hr.ravel()[np.flatnonzero(problem.present(hr).ravel())],
+         "Ys": std.ravel()[np.flatnonzero(problem.present(std).ravel())]}
+    p["W"] = sp.csr_matrix((vals, cols, rp), shape=(nrow, ncol))
+    return p
+
+
+@pytest.mark.parametrize("ratio,dims,precision", [(1, (14, 12, 10), "fp64"), (3, (27, 24, 21), "fp64"),
+                                                  (3, (27, 24, 21), "fp32")])
+def test_w_stencil_layout_other_lattices(gpu_available, ratio, dims, precision):
+    """The stencil layout's tiled interior W^T v and gather W mu on lattices of
+    spacing 1 (same-resolution kriging, one residue class) and 3 (27 classes)
+    against the plain CSR kernels."""
+    from dualres_amd import Chain
+    p = _ratio_problem(dims, ratio, THETA2, radius=2.2 * ratio)
+    outs = []
+    for force_csr in (True, False):
+        c = Chain(p["ijk"], THETA2, p["q_hr"], n_datasets=2, precision=precision, state_mode="real", seed=4)
+        c.w_layout(force_csr)
+        c.set_data(p["Yh"], p["Ys"], p["W"])
+        c.set_sigma_sq_inv([1.1, 0.6])
+        st = c.w_stats()
+        if not force_csr:
+            assert st["stencil_layout"] and st["interior_rows"] > 0 and st["interior_tiles"] > 0
+            assert st["interior_rows"] + st["boundary_rows"] == p["W"].shape[0]
+            assert st["stencil_entries"] * st["interior_rows"] + st["boundary_entries"] == p["W"].nnz
+            if ratio == 3:
+                assert st["far_rows"] > 0  # neighbourhoods that wrap across a row end
+        rng = np.random.default_rng(6)
+        z = (rng.standard_normal(c.N), rng.standard_normal(c.N))
+        outs.append((c.gradient(), c.update(0.05, 3, 0.5, noise=z), c.log_likelihood()))
+        c.close()
+    tol = 1e-13 if precision == "fp64" else 2e-6
+    assert rel(outs[1][0], outs[0][0]) < tol
+    htol = 1e-13 if precision == "fp64" else 1e-5
+    for k in ("H0", "H1"):
+        assert abs(outs[1][1][k] - outs[0][1][k]) <= htol * abs(outs[0][1][k])
+    assert abs(outs[1][2] - outs[0][2]) <= htol * abs(outs[0][2])
