@@ -33,6 +33,7 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 std::vector<int> factorize(int M) {
   std::vector<int> f;
   int m = M;
+  while (m % 16 == 0) { f.push_back(16); m /= 16; }
   while (m % 8 == 0 && m > 8) { f.push_back(8); m /= 8; }
   if (m % 8 == 0) { f.push_back(8); m /= 8; }
   while (m % 4 == 0) { f.push_back(4); m /= 4; }

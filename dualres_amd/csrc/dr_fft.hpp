@@ -103,6 +103,44 @@ template <typename T, bool INV> struct Dft<T, 8, INV> {
   }
 };
 
+// Radix 16 as 4 x 4: with n = 4 n1 + n2 and k = k1 + 4 k2,
+//   X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1).
+struct Tw16 {  // cos and sin of 2 pi j / 16, j = 0..9
+  static constexpr double c[10] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                   0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                   -1.0, -0.92387953251128675613};
+  static constexpr double s[10] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                                   1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                   0.0, -0.38268343236508977173};
+};
+template <typename T, bool INV> struct Dft<T, 16, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> a[4][4];  // a[n2][k1]
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cx<T> t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<T, 4, INV>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) {
+        const int j = n2 * k1;  // W16^j = exp(-+ 2 pi i j / 16)
+        if (j == 0) {
+          a[n2][k1] = t[k1];
+        } else {
+          const cx<T> w = mk<T>((T)Tw16::c[j], INV ? (T)Tw16::s[j] : -(T)Tw16::s[j]);
+          a[n2][k1] = cmul(t[k1], w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cx<T> u[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Dft<T, 4, INV>::run(u);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = u[k2];
+    }
+  }
+};
+
 // Odd radix R = 2m+1 via the symmetric/antisymmetric split:
 //   X_q = x0 + sum_k cos(2 pi kq/R) t_k  -/+ i sum_k sin(2 pi kq/R) u_k
 // with t_k = x_k + x_{R-k}, u_k = x_k - x_{R-k}.
@@ -215,6 +253,7 @@ struct Factors {
 constexpr Factors factor_ct(int M) {  // the radix order of factorize()
   Factors f{};
   int m = M;
+  while (m % 16 == 0) { f.r[f.n++] = 16; m /= 16; }
   while (m % 8 == 0 && m > 8) { f.r[f.n++] = 8; m /= 8; }
   if (m % 8 == 0) { f.r[f.n++] = 8; m /= 8; }
   while (m % 4 == 0) { f.r[f.n++] = 4; m /= 4; }
@@ -287,6 +326,7 @@ __device__ __forceinline__ void fft_lds(cx<T>* lds, int P, int tbs, const AxisFF
   for (int s = 0; s < ax.nst; ++s) {
     const int R = ax.radix[s];
     switch (R) {
+      case 16: fft_stage<T, 16, INV, EPT>(lds, P, tbs, ax, Ns); break;
       case 8: fft_stage<T, 8, INV, EPT>(lds, P, tbs, ax, Ns); break;
       case 4: fft_stage<T, 4, INV, EPT>(lds, P, tbs, ax, Ns); break;
       case 2: fft_stage<T, 2, INV, EPT>(lds, P, tbs, ax, Ns); break;
