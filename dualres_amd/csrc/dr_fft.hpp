@@ -391,7 +391,9 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];
   const int TB = 1 << tbs;
-  const int b0 = blockIdx.x * TB;
+  // XCD-contiguous tiles: neighbouring tiles share the cache lines of the
+  // 4-byte 1/lambda rows, so both halves are fetched into one L2
+  const int b0 = xcd_tile(blockIdx.x, gridDim.x) * TB;
   const int M = MC > 0 ? MC : ax.M;
   for (int idx = threadIdx.x; idx < M * TB; idx += kThreads) {
     const int b = idx & (TB - 1), k = idx >> tbs;
