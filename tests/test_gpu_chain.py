@@ -387,3 +387,34 @@ def test_w_stencil_layout_other_lattices(gpu_available, ratio, dims, precision):
     for k in ("H0", "H1"):
         assert abs(outs[1][1][k] - outs[0][1][k]) <= htol * abs(outs[0][1][k])
     assert abs(outs[1][2] - outs[0][2]) <= htol * abs(outs[0][2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gradient_long_z_embedding(gpu_available, precision):
+    """A 256-long z axis, as in the 1 mm workload, with the z passes pruned to
+    40 planes: gradient, a leapfrog trajectory and every energy of it
+    (reduced over the fused z pass's and the half kicks' tiles) against the
+    oracle."""
+    p = small_single(dims=(6, 6, 40))
+    embed = (16, 16, 256)
+    o, data, c, mu, mom = _setup(p, THETA1, precision, "compat", embed)
+    g_o = o.compute_gradient(data, o.mu)
+    g_e = c.gradient()
+    assert rel(g_e.real, g_o.real) < (1e-10 if precision == "fp64" else 2e-5)
+    o._compute_mass_matrix_eigen_values()
+    eps, L = 0.05, 4
+    lp0 = o._log_posterior(data, mu)
+    k0 = o._potential_energy(mom)
+    mu1, p1 = o.leapfrog(data, mu.copy(), mom.copy(), eps, L)
+    lp1 = o._log_posterior(data, mu1)
+    k1 = o._potential_energy(p1)
+    r = c.leapfrog(eps, L)
+    mu_e, p_e = c.get_state()
+    tol, etol = (1e-9, 1e-9) if precision == "fp64" else (2e-4, 1e-4)
+    assert rel(mu_e.real, mu1.real) < tol
+    assert rel(p_e.real, p1.real) < tol
+    assert rel(mu_e.imag, mu1.imag) < tol
+    assert abs((r["log_prior1"] + r["log_lik1"]) - lp1) < etol * abs(lp1)
+    assert abs((r["log_prior0"] + r["log_lik0"]) - lp0) < etol * abs(lp0)
+    assert abs(r["k1"] - k1) < etol * abs(k1)
+    assert abs(r["e0"] - k0) < etol * abs(k0)
