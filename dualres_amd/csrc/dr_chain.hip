@@ -210,12 +210,12 @@ struct Chain : public ChainBase {
   int64_t nsum = 0;
   // stencil + boundary W (see build_stencil)
   struct WStencil {
-    DevBuf<int> irow, ibase, ilidx, doff, sub_start, sub_ldo, tiles;
-    DevBuf<T> w, sub_w, Vl;
+    DevBuf<int> irow, ibase, ilidx, sub_start, sub_ldo, tiles, mpar, wstart, woff;
+    DevBuf<T> sub_w, Vl, ww;
     DevBuf<int> brow, brp, bbase, banchor, tg, trp, frow, frp, fcol, tfp, tfidx;
     DevBuf<uint16_t> bd, td;
     DevBuf<T> bw, tw, Vb, fw, tfw, Vf;
-    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0, n_far = 0;
+    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0, n_far = 0, n_mslot = 0, ncw = 0;
     // interior W^T v tiles: voxel tile dims, LDS box dims, lattice c + sg Z^3,
     // smallest lattice offset, lattice grid box (lo, lv)
     int n_tiles = 0, ncls = 1, tdim[3] = {1, 1, 1}, ldim[3] = {1, 1, 1}, c[3] = {0, 0, 0}, sg[3] = {1, 1, 1},
@@ -497,8 +497,10 @@ struct Chain : public ChainBase {
     if (r[0] == reg[0] && r[1] == reg[1] && r[2] == reg[2]) return;
     for (int a = 0; a < 3; ++a) reg[a] = r[a];
     nreg = (long)reg[0] * reg[1] * reg[2];
-    RA.alloc(nreg);
-    RB.alloc(nreg);
+    RA.alloc(nreg + 4);  // + 4: the aligned 4-voxel windows of W mu may end past the region
+    RB.alloc(nreg + 4);
+    RA.zero(stream);
+    RB.zero(stream);
     Yg.alloc(nreg);
     mask.alloc(nreg);
     Yg.zero(stream);
@@ -768,7 +770,6 @@ struct Chain : public ChainBase {
     const long nVl = L[0] * L[1] * L[2];
     DR_REQUIRE(nVl < (1L << 31), DR_EDOMAIN, "W stencil lattice too large");
     auto flatL = [&](const long* u) { return (u[0] - lo[0]) + L[0] * ((u[1] - lo[1]) + L[1] * (u[2] - lo[2])); };
-    if ((long)nD * (4 + (long)sizeof(T)) > 64 * 1024) return false;  // LDS table of the interior W mu blocks
     const int ncls = (int)(sg[0] * sg[1] * sg[2]);
     std::vector<int32_t> sub_start(ncls + 1, 0), sub_o(3 * (size_t)nD);
     std::vector<T> sub_w(nD);
@@ -796,6 +797,51 @@ struct Chain : public ChainBase {
       xyz(ibase[k], b);
       for (int a = 0; a < 3; ++a) u[a] = (b[a] - c[a]) / sg[a];
       ilidx[k] = (int32_t)flatL(u);
+    }
+    // 2b. interior W mu by aligned 4-voxel windows: a row whose base b has
+    //     b mod 4 = p reads its stencil as 16-byte-aligned windows (offsets
+    //     off = 4 w - p from b; weight 0 where a window voxel is no stencil
+    //     entry), so one vector load serves up to four entries.  Rows are
+    //     bucketed by p, one parity per workgroup of kThreads / 8 rows.
+    std::vector<int32_t> wstart(5, 0), woff;
+    std::vector<T> ww;
+    for (int p = 0; p < 4; ++p) {
+      std::unordered_map<long, int> at;
+      for (int e = 0; e < nD; ++e) {
+        const long a4 = p + (long)D[e], wdw = fdiv(a4, 4), off = 4 * wdw - p;
+        auto it = at.find(off);
+        int idx = 0;
+        if (it == at.end()) {
+          idx = (int)woff.size();
+          at.emplace(off, idx);
+          woff.push_back((int32_t)off);
+          ww.insert(ww.end(), 4, (T)0);
+        } else {
+          idx = it->second;
+        }
+        ww[4 * (size_t)idx + (size_t)(a4 - 4 * wdw)] = wD[e];
+      }
+      wstart[p + 1] = (int32_t)woff.size();
+    }
+    long ncw = 0;
+    for (int p = 0; p < 4; ++p) ncw = std::max<long>(ncw, wstart[p + 1] - wstart[p]);
+    if (ncw * (4 + 4 * (long)sizeof(T)) > 48 * 1024) return false;  // LDS window table of W mu
+    const int rpb = kThreads / 8;
+    std::vector<int32_t> mrow, mbase, mlidx, mpar;
+    for (int p = 0; p < 4; ++p) {
+      const size_t s0 = mrow.size();
+      for (long k = 0; k < n_int; ++k)
+        if (ibase[k] % 4 == p) {
+          mrow.push_back(irow[k]);
+          mbase.push_back(ibase[k]);
+          mlidx.push_back(ilidx[k]);
+        }
+      while (mrow.size() % rpb) {
+        mrow.push_back(-1);
+        mbase.push_back(0);
+        mlidx.push_back(0);
+      }
+      for (size_t b = s0; b < mrow.size(); b += rpb) mpar.push_back(p);
     }
     // 3. interior W^T v by tiles: voxel tiles T0 x T1 x T2 that hold a data
     //    voxel; each stages the lattice box its voxels reach (LT0 x LT1 x LT2)
@@ -1019,6 +1065,15 @@ struct Chain : public ChainBase {
                                      ((q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10));
         DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: transposed entry outside the region");
       }
+    for (size_t k = 0; k < mrow.size(); ++k) {
+      if (mrow[k] < 0) continue;
+      const int p = mpar[k / rpb];
+      DR_REQUIRE(mbase[k] % 4 == p, DR_ELOGIC, "W stencil: row parity");
+      for (int c = wstart[p]; c < wstart[p + 1]; ++c)
+        DR_REQUIRE((long)mbase[k] + woff[c] >= 0 && (long)mbase[k] + woff[c] + 3 < nreg + 4 &&
+                       ((long)mbase[k] + woff[c]) % 4 == 0,
+                   DR_ELOGIC, "W stencil: window outside the padded region or unaligned");
+    }
     for (size_t t = 0; t < fcol.size(); ++t)
       DR_REQUIRE(fcol[t] >= 0 && fcol[t] < nreg, DR_ELOGIC, "W stencil: far-row entry outside the region");
     for (size_t t = 0; t < tfidx.size(); ++t)
@@ -1043,11 +1098,15 @@ struct Chain : public ChainBase {
       ws.lv[a] = (int)L[a];
     }
     ws.ncls = ncls;
-    upload(ws.irow, irow);
-    upload(ws.ibase, ibase);
-    upload(ws.ilidx, ilidx);
-    upload(ws.doff, D);
-    upload(ws.w, wD);
+    upload(ws.irow, mrow);
+    upload(ws.ibase, mbase);
+    upload(ws.ilidx, mlidx);
+    upload(ws.mpar, mpar);
+    upload(ws.wstart, wstart);
+    upload(ws.woff, woff);
+    upload(ws.ww, ww);
+    ws.n_mslot = (int)mrow.size();
+    ws.ncw = (int)ncw;
     upload(ws.sub_start, sub_start);
     upload(ws.sub_ldo, sub_ldo);
     upload(ws.sub_w, sub_w);
@@ -1102,7 +1161,7 @@ struct Chain : public ChainBase {
     DR_HIP(hipMemcpyAsync(Yg.p, yg.data(), nreg * sizeof(T), hipMemcpyHostToDevice, stream));
     DR_HIP(hipMemcpyAsync(mask.p, mk_.data(), nreg, hipMemcpyHostToDevice, stream));
     if (cfg.n_datasets == 2) {
-      const long need = cdiv(n_s, kThreads / 64) + cdiv(n_s * 8, kThreads) + 2;
+      const long need = cdiv(n_s, kThreads / 64) + cdiv(n_s * 8, kThreads) + 8;  // + parity-bucket padding
       if (need > pstride) {
         pstride = need;
         part.alloc((size_t)NSLOT * pstride);
@@ -1194,16 +1253,17 @@ struct Chain : public ChainBase {
     const int g1 = cdiv(n_s, kThreads / 64);
     const int g2 = cdiv(n_h, kThreads / 16);
     if (use_cls) {
-      const int gi = cdiv((long)ws.n_int * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
+      const int gi = cdiv((long)ws.n_mslot * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
                 gf = cdiv(ws.n_far, kThreads / 64);
       if (slot >= 0) note_blocks(slot, gi + gb + gf);
       const int R0 = reg[0], R01 = reg[0] * reg[1];
       if (gi + gb + gf > 0) {
-        const WmuInt<T> ai{ws.n_int, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.nD, ws.doff.p, ws.w.p, ws.Vl.p};
+        const WmuInt<T> ai{ws.n_mslot, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.mpar.p, ws.wstart.p, ws.woff.p,
+                           ws.ww.p, ws.ncw, ws.Vl.p};
         const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01,
                            ws.Vb.p};
         const WmuFar<T> af{ws.n_far, ws.frow.p, ws.frp.p, ws.fcol.p, ws.fw.p, ws.Vf.p};
-        hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb + gf), dim3(kThreads), (size_t)ws.nD * (4 + sizeof(T)), stream,
+        hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb + gf), dim3(kThreads), (size_t)ws.ncw * (4 + 4 * sizeof(T)), stream,
                            ai, ab, af, gi, gb, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
         DR_HIP(hipGetLastError());
       }

@@ -394,7 +394,8 @@ __device__ __forceinline__ float from_bits(uint32_t u) { return __uint_as_float(
 __device__ __forceinline__ double from_bits(uint64_t u) { return __longlong_as_double((long long)u); }
 
 template <typename T> struct WmuInt {
-  int n; const int* row; const int* base; const int* lidx; int nD; const int* doff; const T* w; T* Vl;
+  int n; const int* row; const int* base; const int* lidx; const int* bpar; const int* wstart; const int* woff;
+  const T* ww; int ncw; T* Vl;
 };
 // far rows: explicit region indices, v to the compact array Vf
 template <typename T> struct WmuFar {
@@ -414,44 +415,64 @@ template <typename T> struct WtvBnd {
   const int* fp; const int* fidx; const T* fw; const T* Vf;  // far-row entries per voxel
 };
 
-// interior rows: 8 lanes per row, lane q takes stencil entries q, q+8, ...
-// (consecutive lanes read consecutive entries of an x-run) from the stencil
-// table staged in LDS (nD * (4 + sizeof(T)) bytes), issuing batches of 8
-// independent gathers before their multiply-adds; the 8 partial sums are
-// combined by a fixed butterfly.  Returns this lane's share of ||Ys - W R||^2.
+// four consecutive values from a 16-byte (float) / 32-byte (double) aligned address
+__device__ __forceinline__ void ld4(const float* p, float* v) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void ld4(const double* p, double* v) {
+  const double2 q0 = reinterpret_cast<const double2*>(p)[0], q1 = reinterpret_cast<const double2*>(p)[1];
+  v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+}
+
+// Interior rows: 8 lanes per row; the workgroup's rows share base parity p
+// (base mod 4), whose table of aligned 4-voxel windows (offset, 4 weights) is
+// staged in LDS.  Lane q takes windows q, q + 8, ... with one vector load
+// each; the 8 partial sums meet in a butterfly.
 template <typename T>
 __device__ __forceinline__ double wmu_int_block(const WmuInt<T>& a, int blk, const T* __restrict__ R,
                                                 const T* __restrict__ Ys, T sig_s) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* sw = reinterpret_cast<T*>(lds_raw);
-  int* so = reinterpret_cast<int*>(lds_raw + (size_t)a.nD * sizeof(T));
-  for (int s = threadIdx.x; s < a.nD; s += kThreads) {
-    sw[s] = a.w[s];
-    so[s] = a.doff[s];
+  int* so = reinterpret_cast<int*>(lds_raw + (size_t)a.ncw * 4 * sizeof(T));
+  const int p = a.bpar[blk];
+  const int cs = a.wstart[p], nc = a.wstart[p + 1] - cs;
+  for (int i = threadIdx.x; i < nc; i += kThreads) {
+    so[i] = a.woff[cs + i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sw[4 * i + j] = a.ww[4 * (cs + i) + j];
   }
   __syncthreads();
   const int q = threadIdx.x & 7;
   const int k = (blk * kThreads + (int)threadIdx.x) >> 3;
+  const int row = k < a.n ? a.row[k] : -1;
   T acc = (T)0;
-  if (k < a.n) {
+  if (row >= 0) {
     const T* Rb = R + a.base[k];
-    constexpr int B = 8;
-    int s = q;
-    for (; s + 8 * (B - 1) < a.nD; s += 8 * B) {
-      T v[B];
+    constexpr int B = 4;
+    int c = q;
+    for (; c + 8 * (B - 1) < nc; c += 8 * B) {
+      T v[B][4];
 #pragma unroll
-      for (int b = 0; b < B; ++b) v[b] = Rb[so[s + 8 * b]];
+      for (int b = 0; b < B; ++b) ld4(Rb + so[c + 8 * b], v[b]);
 #pragma unroll
-      for (int b = 0; b < B; ++b) acc += sw[s + 8 * b] * v[b];
+      for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc += sw[4 * (c + 8 * b) + j] * v[b][j];
     }
-    for (; s < a.nD; s += 8) acc += sw[s] * Rb[so[s]];
+    for (; c < nc; c += 8) {
+      T v[4];
+      ld4(Rb + so[c], v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc += sw[4 * c + j] * v[j];
+    }
   }
   acc += __shfl_xor(acc, 1, 8);
   acc += __shfl_xor(acc, 2, 8);
   acc += __shfl_xor(acc, 4, 8);
   double e2 = 0.0;
-  if (k < a.n && q == 0) {
-    const T e = Ys[a.row[k]] - acc;
+  if (row >= 0 && q == 0) {
+    const T e = Ys[row] - acc;
     a.Vl[a.lidx[k]] = sig_s * e;
     e2 = (double)e * (double)e;
   }
