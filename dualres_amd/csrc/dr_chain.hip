@@ -212,10 +212,10 @@ struct Chain : public ChainBase {
   struct WStencil {
     DevBuf<int> irow, ibase, ilidx, sub_start, sub_ldo, tiles, mpar, wstart, woff;
     DevBuf<T> sub_w, Vl, ww;
-    DevBuf<int> brow, brp, bbase, banchor, tg, trp, frow, frp, fcol, tfp, tfidx;
+    DevBuf<int> brow, brp, bbase, tbase, tg, trp, frow, frp, fcol, tfp, tfidx;
     DevBuf<uint16_t> bd, td;
-    DevBuf<T> bw, tw, Vb, fw, tfw, Vf;
-    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, dc = 0, n_far = 0, n_mslot = 0, ncw = 0;
+    DevBuf<T> bw, tw, Vc, fw, tfw, Vf;
+    int n_int = 0, nD = 0, n_bnd = 0, n_touch = 0, n_far = 0, n_mslot = 0, ncw = 0;
     // interior W^T v tiles: voxel tile dims, LDS box dims, lattice c + sg Z^3,
     // smallest lattice offset, lattice grid box (lo, lv)
     int n_tiles = 0, ncls = 1, tdim[3] = {1, 1, 1}, ldim[3] = {1, 1, 1}, c[3] = {0, 0, 0}, sg[3] = {1, 1, 1},
@@ -645,9 +645,10 @@ struct Chain : public ChainBase {
 
   // Stencil + boundary layout of W (interior and boundary blocks of k_wmu_sb
   // and k_wtv_sb).  cols: image-region grid index of every entry.  Returns
-  // false (plain CSR kernels) when a boundary row spans more than 16 voxels on
-  // an axis, has no free anchor position, or the stencil is too wide for the
-  // LDS table of W mu or the lattice box of a W^T v tile.
+  // false (plain CSR kernels) when the boundary rows touching one voxel lie
+  // more than 65535 rows apart, or the stencil is too wide for the LDS table
+  // of W mu or the lattice box of a W^T v tile.  Boundary rows wider than 16
+  // voxels on an axis become far rows.
   // Entry order for the quad loads of the boundary blocks: a list
   // [start, end) (a multiple of 4) is cut into blocks of `lanes` quads; in a
   // block of nq quads, quad l holds entries l, l + nq, l + 2 nq, l + 3 nq, so
@@ -878,16 +879,15 @@ struct Chain : public ChainBase {
     std::vector<int32_t> sub_ldo(nD);
     for (int e = 0; e < nD; ++e)
       sub_ldo[e] = (int32_t)(sub_o[3 * e] + LT[0] * (sub_o[3 * e + 1] + LT[1] * sub_o[3 * e + 2]));
-    // 4. boundary rows: packed offsets from the row's bounding-box corner and
-    //    a distinct anchor position (one of the row's own entries) for v.
-    //    Rows that span more than 16 voxels on an axis (the reference's
-    //    flat-offset neighbourhood can wrap across a row end) or find no free
-    //    anchor are far rows: explicit region indices, v in a compact array.
-    std::vector<int32_t> brow, brp{0}, bbase, banchor;
+    // 4. boundary rows: packed offsets from the row's bounding-box corner; v
+    //    of boundary row k goes to Vc[k], a compact array.  Rows that span
+    //    more than 16 voxels on an axis (the reference's flat-offset
+    //    neighbourhood can wrap across a row end) are far rows: explicit
+    //    region indices, v in a second compact array.
+    std::vector<int32_t> brow, brp{0}, bbase;
     std::vector<uint16_t> bd;
     std::vector<T> bw;
-    std::vector<uint8_t> claimed(nreg, 0);
-    std::vector<int32_t> row_anchor(n_s, -1), row_far(n_s, -1);
+    std::vector<int32_t> row_bnd(n_s, -1), row_far(n_s, -1);
     std::vector<int32_t> frow, frp{0}, fcol;
     std::vector<T> fwt;
     long nb_real = 0;
@@ -901,17 +901,10 @@ struct Chain : public ChainBase {
           mx[a] = std::max(mx[a], x[a]);
         }
       }
-      int32_t an = -1;
       if (rp[i + 1] > rp[i]) {
         bool wide = false;
         for (int a = 0; a < 3; ++a) wide = wide || mx[a] - mn[a] > 15;
-        if (!wide)
-          for (int32_t t = rp[i]; t < rp[i + 1]; ++t)
-            if (!claimed[cols[t]]) {
-              an = cols[t];
-              break;
-            }
-        if (an < 0) {  // far row
+        if (wide) {  // far row
           row_far[i] = (int32_t)frow.size();
           frow.push_back((int32_t)i);
           for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
@@ -922,9 +915,8 @@ struct Chain : public ChainBase {
           nb_real += rp[i + 1] - rp[i];
           continue;
         }
-        claimed[an] = 1;
       }
-      row_anchor[i] = an;
+      row_bnd[i] = (int32_t)brow.size();
       const long base = rp[i + 1] > rp[i] ? mn[0] + R0 * mn[1] + R01 * mn[2] : 0;
       nb_real += rp[i + 1] - rp[i];
       for (int32_t t = rp[i]; t < rp[i + 1]; ++t) {
@@ -939,11 +931,11 @@ struct Chain : public ChainBase {
       interleave_quads(bd, bw, brp.back(), (long)bw.size(), 64);
       brow.push_back((int32_t)i);
       bbase.push_back((int32_t)base);
-      banchor.push_back(an);
       brp.push_back((int32_t)bw.size());
     }
     // 5. transposed boundary entries per touched data voxel, in row order:
-    //    packed quads from the boundary rows, then (far row, weight) pairs
+    //    quads of (boundary row - the voxel's first boundary row, weight) as
+    //    16-bit deltas into Vc, then (far row, weight) pairs
     std::vector<int32_t> tcount(n_h + 1, 0), fcount(n_h + 1, 0);
     for (long i = 0; i < n_s; ++i)
       if (!interior[i])
@@ -951,23 +943,19 @@ struct Chain : public ChainBase {
     std::vector<int32_t> tstart(n_h + 1, 0);
     for (long d = 0; d < n_h; ++d) tstart[d + 1] = tstart[d] + ((tcount[d + 1] + 3) & ~3);
     const long nbt = tstart[n_h];
-    const uint16_t qself = (uint16_t)(15 | (15 << 5) | (15 << 10));  // offset 0: the voxel itself
-    std::vector<uint16_t> tdall(nbt, qself);
+    std::vector<uint16_t> tdall(nbt, 0);  // pad: delta 0, weight 0
     std::vector<T> twall(nbt, (T)0);
+    std::vector<int32_t> vbase(n_h, -1);  // first boundary row touching each voxel
     {
       std::vector<int32_t> fill(tstart.begin(), tstart.end() - 1);
       for (long i = 0; i < n_s; ++i) {
-        if (interior[i] || row_far[i] >= 0) continue;
-        long xa[3], x[3];
-        xyz(row_anchor[i] >= 0 ? row_anchor[i] : 0, xa);
+        if (row_bnd[i] < 0) continue;
         for (long t = W_rp[i]; t < W_rp[i + 1]; ++t) {
           const int32_t dv = W_cols[t];
-          xyz(gidx[dv], x);
-          long q[3];
-          for (int a = 0; a < 3; ++a) q[a] = x[a] - xa[a] + 15;
-          for (int a = 0; a < 3; ++a)
-            DR_REQUIRE(q[a] >= 0 && q[a] <= 31, DR_ELOGIC, "W stencil: transposed offset out of range");
-          tdall[fill[dv]] = (uint16_t)(q[0] | (q[1] << 5) | (q[2] << 10));
+          if (vbase[dv] < 0) vbase[dv] = row_bnd[i];  // rows come in increasing order
+          const long delta = (long)row_bnd[i] - vbase[dv];
+          if (delta > 65535) return false;  // rows of one voxel too far apart for 16-bit deltas
+          tdall[fill[dv]] = (uint16_t)delta;
           twall[fill[dv]] = vals[t];
           fill[dv]++;
         }
@@ -990,10 +978,11 @@ struct Chain : public ChainBase {
         }
       }
     }
-    std::vector<int32_t> tg, trp{0}, tfp{0}, tfp_start;
+    std::vector<int32_t> tg, trp{0}, tfp{0}, tfp_start, tbase;
     for (long d = 0; d < n_h; ++d)
       if (tstart[d + 1] > tstart[d] || fstart[d + 1] > fstart[d]) {
         tg.push_back(gidx[d]);
+        tbase.push_back(std::max(vbase[d], 0));
         trp.push_back(tstart[d + 1]);
         tfp_start.push_back(fstart[d]);
         tfp.push_back(fstart[d + 1]);
@@ -1053,18 +1042,17 @@ struct Chain : public ChainBase {
                      "W stencil: sub-stencil offset outside the box margin");
     }
     for (size_t k = 0; k < brow.size(); ++k) {
-      DR_REQUIRE(banchor[k] < nreg, DR_ELOGIC, "W stencil: anchor outside the region");
       for (int32_t t = brp[k]; t < brp[k + 1]; ++t) {
         const long q = bd[t], j = (long)bbase[k] + (q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10);
         DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: boundary entry outside the region");
       }
     }
-    for (size_t k = 0; k < tg.size(); ++k)
-      for (int32_t t = trp[k]; t < trp[k + 1]; ++t) {
-        const long q = tdall[t], j = (long)tg[k] + (15 + 15 * R0 + 15 * R01) -
-                                     ((q & 31) + R0 * ((q >> 5) & 31) + R01 * (q >> 10));
-        DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: transposed entry outside the region");
-      }
+    for (size_t k = 0; k < tg.size(); ++k) {
+      DR_REQUIRE(tg[k] >= 0 && tg[k] < nreg, DR_ELOGIC, "W stencil: touched voxel outside the region");
+      for (int32_t t = trp[k]; t < trp[k + 1]; ++t)
+        DR_REQUIRE((long)tbase[k] + tdall[t] < std::max<long>((long)brow.size(), 1), DR_ELOGIC,
+                   "W stencil: transposed entry outside the boundary rows");
+    }
     for (size_t k = 0; k < mrow.size(); ++k) {
       if (mrow[k] < 0) continue;
       const int p = mpar[k / rpb];
@@ -1086,7 +1074,6 @@ struct Chain : public ChainBase {
     ws.nD = nD;
     ws.n_bnd = (int)brow.size();
     ws.n_touch = (int)tg.size();
-    ws.dc = (int)(15 + 15 * R0 + 15 * R01);
     ws.n_tiles = (int)(tiles.size() / 3);
     for (int a = 0; a < 3; ++a) {
       ws.tdim[a] = (int)Tt[a];
@@ -1114,7 +1101,7 @@ struct Chain : public ChainBase {
     upload(ws.brow, brow);
     upload(ws.brp, brp);
     upload(ws.bbase, bbase);
-    upload(ws.banchor, banchor);
+    upload(ws.tbase, tbase);
     upload(ws.bd, bd);
     upload(ws.bw, bw);
     upload(ws.tg, tg);
@@ -1132,8 +1119,8 @@ struct Chain : public ChainBase {
     ws.Vf.zero(stream);
     ws.Vl.alloc(std::max<long>(nVl, 1));
     ws.Vl.zero(stream);
-    ws.Vb.alloc(std::max<long>(nreg, 1));
-    ws.Vb.zero(stream);
+    ws.Vc.alloc(std::max<long>((long)brow.size(), 1));
+    ws.Vc.zero(stream);
     DR_HIP(hipStreamSynchronize(stream));
     wstats[0] = n_int;
     wstats[1] = nD;
@@ -1260,8 +1247,7 @@ struct Chain : public ChainBase {
       if (gi + gb + gf > 0) {
         const WmuInt<T> ai{ws.n_mslot, ws.irow.p, ws.ibase.p, ws.ilidx.p, ws.mpar.p, ws.wstart.p, ws.woff.p,
                            ws.ww.p, ws.ncw, ws.Vl.p};
-        const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, ws.banchor.p, R0, R01,
-                           ws.Vb.p};
+        const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, R0, R01, ws.Vc.p};
         const WmuFar<T> af{ws.n_far, ws.frow.p, ws.frp.p, ws.fcol.p, ws.fw.p, ws.Vf.p};
         hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb + gf), dim3(kThreads), (size_t)ws.ncw * (4 + 4 * sizeof(T)), stream,
                            ai, ab, af, gi, gb, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
@@ -1288,7 +1274,7 @@ struct Chain : public ChainBase {
         wi.lv[a] = ws.lv[a];
         wi.R[a] = reg[a];
       }
-      const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.td.p, ws.tw.p, R0, R01, ws.dc, ws.Vb.p, WTvb.p,
+      const WtvBnd<T> wb{ws.n_touch, ws.tg.p, ws.trp.p, ws.tbase.p, ws.td.p, ws.tw.p, ws.Vc.p, WTvb.p,
                          ws.tfp.p, ws.tfidx.p, ws.tfw.p, ws.Vf.p};
       const size_t lds_t = (size_t)ws.ldim[0] * ws.ldim[1] * ws.ldim[2] * sizeof(T);
       hipLaunchKernelGGL(k_wtv_sb<T>, dim3(ti + tb), dim3(kThreads), lds_t, stream, wi, wb, ti);

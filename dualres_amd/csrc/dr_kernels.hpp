@@ -401,17 +401,19 @@ template <typename T> struct WmuInt {
 template <typename T> struct WmuFar {
   int n; const int* row; const int* rp; const int* col; const T* w; T* Vf;
 };
+// boundary rows: packed offsets from each row's base voxel, v of row k to Vc[k]
 template <typename T> struct WmuBnd {
-  int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w; const int* anchor;
-  int R0, R01; T* Vb;
+  int n; const int* row; const int* rp; const int* base; const uint16_t* d; const T* w;
+  int R0, R01; T* Vc;
 };
 template <typename T> struct WtvInt {
   const int* tiles; const int* cls_start; const int* ldo; const typename Bits<T>::type* w; const T* Vl;
   const uint8_t* mask; T* out;
   int ncls, TD[3], LT[3], c[3], sg[3], omin[3], lo[3], lv[3], R[3];
 };
+// touched voxels: Vc index = base[k] + 16-bit delta per entry
 template <typename T> struct WtvBnd {
-  int n; const int* g; const int* rp; const uint16_t* d; const T* w; int R0, R01, dc; const T* Vb; T* out;
+  int n; const int* g; const int* rp; const int* base; const uint16_t* d; const T* w; const T* Vc; T* out;
   const int* fp; const int* fidx; const T* fw; const T* Vf;  // far-row entries per voxel
 };
 
@@ -504,8 +506,7 @@ __device__ __forceinline__ double wmu_bnd_block(const WmuBnd<T>& a, int blk, con
     acc = wave_sum(acc);
     if (lane == 0) {
       const T e = Ys[a.row[k]] - acc;
-      const int an = a.anchor[k];
-      if (an >= 0) a.Vb[an] = sig_s * e;
+      a.Vc[k] = sig_s * e;
       e2 = (double)e * (double)e;
     }
   }
@@ -659,9 +660,9 @@ __device__ __forceinline__ void wtv_int_block(const WtvInt<T>& a, int blk) {
 
 // W^T v of the boundary and far rows at the voxels they touch (written to its
 // own field, which the residual adds); kWtvLanes lanes per voxel, quads of
-// entries as in the W mu part (pad offset = the voxel itself, weight 0), then
-// the voxel's far-row entries (far row index, weight).  Quad offsets are
-// voxel - anchor + 15 per axis, packed.
+// entries (boundary row - the voxel's first boundary row, as 16-bit deltas
+// into the compact boundary v; pads are delta 0, weight 0), then the voxel's
+// far-row entries (far row index, weight).
 template <typename T>
 __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
   const int lane = threadIdx.x & (kWtvLanes - 1);
@@ -671,16 +672,16 @@ __device__ __forceinline__ void wtv_bnd_block(const WtvBnd<T>& a, int blk) {
   if (k < a.n) {
     gj = a.g[k];
     const int j1 = a.rp[k + 1] >> 2;
-    const T* Vg = a.Vb + (gj + a.dc);
+    const T* Vg = a.Vc + a.base[k];
     const uint2* d4 = reinterpret_cast<const uint2*>(a.d);
     const Quad<T>* w4 = reinterpret_cast<const Quad<T>*>(a.w);
     for (int j = (a.rp[k] >> 2) + lane; j < j1; j += kWtvLanes) {
       const uint2 dq = d4[j];
       const Quad<T> wq = w4[j];
-      acc += wq.v[0] * Vg[-unpack3(lo16(dq.x), a.R0, a.R01)];
-      acc += wq.v[1] * Vg[-unpack3(hi16(dq.x), a.R0, a.R01)];
-      acc += wq.v[2] * Vg[-unpack3(lo16(dq.y), a.R0, a.R01)];
-      acc += wq.v[3] * Vg[-unpack3(hi16(dq.y), a.R0, a.R01)];
+      acc += wq.v[0] * Vg[lo16(dq.x)];
+      acc += wq.v[1] * Vg[hi16(dq.x)];
+      acc += wq.v[2] * Vg[lo16(dq.y)];
+      acc += wq.v[3] * Vg[hi16(dq.y)];
     }
     for (int j = a.fp[k] + lane; j < a.fp[k + 1]; j += kWtvLanes) acc += a.fw[j] * a.Vf[a.fidx[j]];
   }
