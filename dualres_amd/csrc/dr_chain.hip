@@ -1321,6 +1321,15 @@ struct Chain : public ChainBase {
     const long sk = (long)M[1] * H0;
     with_fft_len(ad.M, [&](auto mc) {
       constexpr int MC = decltype(mc)::value;
+      // state prefetch for lengths whose 16 per-thread modes fit in registers
+      if constexpr (MC > 0 && MC <= 256 && MC % (kThreads / 16) == 0) {
+        if (TB == 16) {
+          hipLaunchKernelGGL((k_strided2_pf<T, LoadS<T>, MidLeapfrog<T>, StoreS<T>, kEptStrided, MC>),
+                             dim3(grid), dim3(kThreads), lds, stream, ad, NB, reg[2], reg[2],
+                             LoadS<T>{{}, T1.p, 0, sk}, mid, StoreS<T>{{}, T1.p, 0, sk});
+          return;
+        }
+      }
       hipLaunchKernelGGL((k_strided2<T, LoadS<T>, MidLeapfrog<T>, StoreS<T>, kEptStrided, MC>), dim3(grid),
                          dim3(kThreads), lds, stream, ad, NB, reg[2], reg[2], tbs, LoadS<T>{{}, T1.p, 0, sk}, mid,
                          StoreS<T>{{}, T1.p, 0, sk});

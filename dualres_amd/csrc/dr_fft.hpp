@@ -419,6 +419,57 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
   epi.finish(red, blk);
 }
 
+// k_strided2 for 16-column tiles of a compile-time length MC: each thread
+// fetches the state of the first NP of its MC * 16 / kThreads modes
+// (Mid::fetch) before the tile's load and forward transform, so that those
+// loads are in flight during the transform (the rest are fetched when
+// updated; NP bounds the registers held across the transform).  The modes
+// are the ones k_strided2's Mid loop gives the thread: column b = thread
+// mod 16, rows thread / 16 + e * kThreads / 16.
+template <typename T, class Pro, class Mid, class Epi, int EPT, int MC>
+__global__ void __launch_bounds__(kThreads)
+k_strided2_pf(AxisFFT<T> ax, int NB, int k_in, int k_out, Pro pro, Mid mid, Epi epi) {
+  constexpr int TB = 16, tbs = 4, KS = kThreads / TB, NE = MC / KS;
+  constexpr int NP = NE < 32 / (int)sizeof(T) ? NE : 32 / (int)sizeof(T);
+  static_assert(NE * KS == MC, "MC must be a multiple of kThreads / 16");
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  __shared__ double red[kThreads];
+  const int b0 = xcd_tile(blockIdx.x, gridDim.x) * TB;
+  const int bt = threadIdx.x & (TB - 1), kt = threadIdx.x >> tbs;
+  const bool in = b0 + bt < NB;
+  const int bf = in ? b0 + bt : b0;  // b0 < NB: the clamped fetch stays in bounds
+  typename Mid::State st[NP];
+#pragma unroll
+  for (int e = 0; e < NP; ++e) st[e] = mid.fetch(kt + e * KS, bf);
+  for (int idx = threadIdx.x; idx < MC * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    cx<T> v = mk<T>(0, 0);
+    if (k < k_in && b0 + b < NB) v = pro.load(0, k, b0 + b);
+    lds[k * TB + b] = v;
+  }
+  __syncthreads();
+  fft_run<T, false, EPT, MC, 1>(lds, TB, tbs, ax);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int k = kt + e * KS;
+    if (in) {
+      if (e < NP) lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt], st[e < NP ? e : 0]);
+      else lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt]);
+    }
+  }
+  __syncthreads();
+  fft_run<T, true, EPT, MC, 1>(lds, TB, tbs, ax);
+  for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
+    const int b = idx & (TB - 1), k = idx >> tbs;
+    if (b0 + b < NB) epi.store(0, k, b0 + b, lds[k * TB + b]);
+  }
+  const int blk = blockIdx.x;
+  pro.finish(red, blk);
+  mid.finish(red, blk);
+  epi.finish(red, blk);
+}
+
 // Real rows of length M0 (first n_in samples nonzero) -> M0/2+1 complex.
 // ax describes the complex FFT of length Mh = M0/2 with tws = 2 into the
 // length-M0 table, so ax.tw[k] = exp(-2 pi i k / M0).

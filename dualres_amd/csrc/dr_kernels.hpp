@@ -97,18 +97,24 @@ template <typename T> struct EpiMomentum {
 template <typename T> struct MidLeapfrog {
   const cx<T>* X; cx<T>* Xd; cx<T>* p; const T* invlam; long sk; T coef; T eps; T sig; int H0;
   int do_red; double* part; double a0;
-  __device__ __forceinline__ cx<T> apply(int k, int b, cx<T> v) {
+  // the state one mode's update reads; each mode is read and written by one
+  // thread only, so a thread may fetch its modes' state early
+  struct State { cx<T> x, p; T il; };
+  __device__ __forceinline__ State fetch(int k, int b) const {
     const long i = (long)k * sk + b;
-    const T il = invlam[i];
-    const cx<T> x = X[i];
-    const cx<T> pn = p[i] + (v - x * il) * coef;
+    return State{X[i], p[i], invlam[i]};
+  }
+  __device__ __forceinline__ cx<T> apply(int k, int b, cx<T> v, const State& s) {
+    const long i = (long)k * sk + b;
+    const cx<T> pn = s.p + (v - s.x * s.il) * coef;
     p[i] = pn;
-    const T c = il > (T)0 ? eps / (il + sig) : (T)0;
-    const cx<T> m = x + pn * c;
+    const T c = s.il > (T)0 ? eps / (s.il + sig) : (T)0;
+    const cx<T> m = s.x + pn * c;
     Xd[i] = m;
-    if (do_red) a0 += herm_w(b % H0, H0) * (double)il * ((double)x.x * x.x + (double)x.y * x.y);
+    if (do_red) a0 += herm_w(b % H0, H0) * (double)s.il * ((double)s.x.x * s.x.x + (double)s.x.y * s.x.y);
     return m;
   }
+  __device__ __forceinline__ cx<T> apply(int k, int b, cx<T> v) { return apply(k, b, v, fetch(k, b)); }
   __device__ __forceinline__ void finish(double* red, int blk) {
     if (do_red) red_write(a0, red, part, blk);
   }
