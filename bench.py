@@ -119,9 +119,10 @@ def kernel_bytes(name, p, prec_bytes, c):
         return T1 * cs + T2 * cs
     if name == "c2r_x":
         return T2 * cs + nreg * s
-    if name == "r2c_x_residual":
-        return nreg * (3 * s + 1) + T2 * cs             # mu, Yh, W^T v, mask -> T2
     ws = c.w_stats()
+    if name == "r2c_x_residual":                        # mu, Yh, W^T v fields, mask -> T2
+        nf = 4 if ws["stencil_layout"] else 3           # the stencil layout keeps W^T v in two fields
+        return nreg * (nf * s + 1) + T2 * cs
     if ws["stencil_layout"]:
         ni, nd, nb, eb = ws["interior_rows"], ws["stencil_entries"], ws["boundary_rows"], ws["boundary_entries"]
         nt = ws["touched_voxels"]

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "dr_common.hpp"
@@ -352,6 +353,31 @@ __device__ __forceinline__ void fft_run(cx<T>* lds, int P, int tbs, const AxisFF
 //   c2r_x:    cx<T> Pro::load(int row, int k);   void Epi::store2(int row, int n, cx<T> z)  (samples 2n, 2n+1)
 // and void finish(double* red, int block) on both for optional reductions.
 
+// Prologues that split into a side-effect-free fetch and an apply
+// (kSplitLoad) fill a tile in groups of kFillGroup elements per thread: all
+// loads of a group are issued before its first apply and LDS write, so a
+// thread waits once per group instead of once per element (or more: the
+// residual's mask decided which loads ran).  Plain single loads gain nothing
+// from it (the y passes were 1-2 us slower).
+template <class P, class = void> struct is_split_load : std::false_type {};
+template <class P> struct is_split_load<P, std::void_t<decltype(P::kSplitLoad)>> : std::bool_constant<P::kSplitLoad> {};
+constexpr int kFillGroup = 4;
+// fetch(idx) must be safe for every idx in [0, total); put(idx, v) stores
+// element idx.  Elements are put in increasing idx per thread, as the plain
+// strided loop does, so reductions inside put keep their order.
+template <int G, class Fetch, class Put>
+__device__ __forceinline__ void fill_grouped(int total, Fetch&& fetch, Put&& put) {
+  using V = decltype(fetch(0));
+  for (int i0 = threadIdx.x; i0 < total; i0 += G * kThreads) {
+    V v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = fetch(min(i0 + u * kThreads, total - 1));
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      if (i0 + u * kThreads < total) put(i0 + u * kThreads, v[u]);
+  }
+}
+
 template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
@@ -483,12 +509,25 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
   const int P = TB + 1;
   const int Mh = MC > 0 ? MC : ax.M;
   const int row0 = blockIdx.x * TB;
-  for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
-    const int l = idx / Mh, n = idx - l * Mh;
-    const int row = row0 + l;
-    cx<T> v = mk<T>(0, 0);
-    if (row < nrows) v = pro.load2(row, n);
-    lds[n * P + l] = v;
+  if constexpr (MC > 0 && is_split_load<Pro>::value) {
+    fill_grouped<kFillGroup>(
+        TB * Mh,
+        [&](int idx) {
+          const int l = idx / Mh, n = idx - l * Mh;
+          return pro.fetch2(min(row0 + l, nrows - 1), n);
+        },
+        [&](int idx, const typename Pro::Raw& q) {
+          const int l = idx / Mh, n = idx - l * Mh;
+          lds[n * P + l] = row0 + l < nrows ? pro.apply2(q, n) : mk<T>(0, 0);
+        });
+  } else {
+    for (int idx = threadIdx.x; idx < TB * Mh; idx += kThreads) {
+      const int l = idx / Mh, n = idx - l * Mh;
+      const int row = row0 + l;
+      cx<T> v = mk<T>(0, 0);
+      if (row < nrows) v = pro.load2(row, n);
+      lds[n * P + l] = v;
+    }
   }
   __syncthreads();
   fft_run<T, false, EPT, MC, 2>(lds, P, tbs, ax);

@@ -262,22 +262,41 @@ template <typename T> struct LoadNoiseBuf {
 // voxels, 0 elsewhere; accumulates ||Yh - mu[I]||^2 (:502-503).
 template <typename T> struct LoadResidual {
   const T* R; const T* Yg; const uint8_t* mask; const T* WTv; const T* WTvb; T sig; int n0; double* part; double acc;
-  __device__ __forceinline__ T one(long g) {
-    if (!mask[g]) return (T)0;
-    const T d = Yg[g] - R[g];
+  // split load: fetch2 reads the inputs of samples 2n, 2n+1 of a row without
+  // side effects (any row < nrows, any n: x is clamped into the row), apply2
+  // forms the residual and accumulates the SSE for the samples inside the row
+  struct Raw { T y[2], r[2], w[2], wb[2]; uint8_t m[2]; };
+  static constexpr bool kSplitLoad = true;
+  __device__ __forceinline__ Raw fetch2(int row, int n) const {
+    Raw q;
+    const long base = (long)row * n0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const long g = base + min(2 * n + e, n0 - 1);
+      q.m[e] = mask[g];
+      q.y[e] = Yg[g];
+      q.r[e] = R[g];
+      q.w[e] = WTv ? WTv[g] : (T)0;
+      q.wb[e] = WTvb ? WTvb[g] : (T)0;
+    }
+    return q;
+  }
+  __device__ __forceinline__ T one(const Raw& q, int e) {
+    if (!q.m[e]) return (T)0;
+    const T d = q.y[e] - q.r[e];
     acc += (double)d * (double)d;
     T r = sig * d;
-    if (WTv) r += WTv[g];
-    if (WTvb) r += WTvb[g];
+    if (WTv) r += q.w[e];
+    if (WTvb) r += q.wb[e];
     return r;
   }
-  __device__ __forceinline__ cx<T> load2(int row, int n) {
+  __device__ __forceinline__ cx<T> apply2(const Raw& q, int n) {
     const int x = 2 * n;
-    const long base = (long)row * n0;
-    const T a = x < n0 ? one(base + x) : (T)0;
-    const T b = x + 1 < n0 ? one(base + x + 1) : (T)0;
+    const T a = x < n0 ? one(q, 0) : (T)0;
+    const T b = x + 1 < n0 ? one(q, 1) : (T)0;
     return mk<T>(a, b);
   }
+  __device__ __forceinline__ cx<T> load2(int row, int n) { return apply2(fetch2(row, n), n); }
   __device__ __forceinline__ void finish(double* red, int blk) {
     if (part) red_write(acc, red, part, blk);
   }
