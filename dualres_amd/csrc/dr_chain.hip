@@ -1320,6 +1320,8 @@ struct Chain : public ChainBase {
     const int TB = 1 << tbs;
     const int grid = cdiv(NB, TB);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<T>);
+    // the tile holds ad.M rows, of which the first reg[2] are loaded
+    DR_REQUIRE(reg[2] >= 1 && reg[2] <= ad.M && NB >= 1, DR_ELOGIC, "z pass: bad extents");
     if (mid_slot >= 0) note_blocks(mid_slot, grid);
     const long sk = (long)M[1] * H0;
     with_fft_len(ad.M, [&](auto mc) {

@@ -468,12 +468,12 @@ k_strided2_pf(AxisFFT<T> ax, int NB, int k_in, int k_out, Pro pro, Mid mid, Epi 
   typename Mid::State st[NP];
 #pragma unroll
   for (int e = 0; e < NP; ++e) st[e] = mid.fetch(kt + e * KS, bf);
-  for (int idx = threadIdx.x; idx < MC * TB; idx += kThreads) {
-    const int b = idx & (TB - 1), k = idx >> tbs;
-    cx<T> v = mk<T>(0, 0);
-    if (k < k_in && b0 + b < NB) v = pro.load(0, k, b0 + b);
-    lds[k * TB + b] = v;
-  }
+  // the tile's k_in input rows in groups of 4 loads per thread (lds index =
+  // k * TB + b = idx), then the zero rows; Pro::load has no side effects
+  fill_grouped<4>(
+      k_in * TB, [&](int idx) { return pro.load(0, idx >> tbs, min(b0 + (idx & (TB - 1)), NB - 1)); },
+      [&](int idx, cx<T> v) { lds[idx] = b0 + (idx & (TB - 1)) < NB ? v : mk<T>(0, 0); });
+  for (int idx = k_in * TB + threadIdx.x; idx < MC * TB; idx += kThreads) lds[idx] = mk<T>(0, 0);
   __syncthreads();
   fft_run<T, false, EPT, MC, 1>(lds, TB, tbs, ax);
 #pragma unroll
