@@ -448,8 +448,8 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
 // k_strided2 for 16-column tiles of a compile-time length MC: each thread
 // fetches the state of the first NP of its MC * 16 / kThreads modes
 // (Mid::fetch) before the tile's load and forward transform, so that those
-// loads are in flight during the transform (the rest are fetched when
-// updated; NP bounds the registers held across the transform).  The modes
+// loads are in flight during the transform (NP bounds the registers held
+// across the transform); the rest are fetched in groups of 4 after it.  The modes
 // are the ones k_strided2's Mid loop gives the thread: column b = thread
 // mod 16, rows thread / 16 + e * kThreads / 16.
 template <typename T, class Pro, class Mid, class Epi, int EPT, int MC>
@@ -477,11 +477,20 @@ k_strided2_pf(AxisFFT<T> ax, int NB, int k_in, int k_out, Pro pro, Mid mid, Epi 
   __syncthreads();
   fft_run<T, false, EPT, MC, 1>(lds, TB, tbs, ax);
 #pragma unroll
-  for (int e = 0; e < NE; ++e) {
+  for (int e = 0; e < NP; ++e) {
     const int k = kt + e * KS;
-    if (in) {
-      if (e < NP) lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt], st[e < NP ? e : 0]);
-      else lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt]);
+    if (in) lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt], st[e]);
+  }
+#pragma unroll
+  for (int e0 = NP; e0 < NE; e0 += 4) {
+    typename Mid::State sr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (e0 + u < NE) sr[u] = mid.fetch(kt + (e0 + u) * KS, bf);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = kt + (e0 + u) * KS;
+      if (e0 + u < NE && in) lds[k * TB + bt] = mid.apply(k, b0 + bt, lds[k * TB + bt], sr[u]);
     }
   }
   __syncthreads();
