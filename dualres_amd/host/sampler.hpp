@@ -75,6 +75,12 @@ class HMCParameters {
     return (value_type)std::max(eps_ * U(rng()), eps0_);
   }
   value_type eps_value() const { return (value_type)eps_; }
+  // dual-averaging state (HMCParameters.h:76-86), read by the host tests
+  double eps_bar() const { return eps_bar_; }
+  double h_bar() const { return h_bar_; }
+  double eps_target() const { return eps_target_; }
+  bool eps_start_found() const { return eps_start_found_; }
+  bool warmup() const { return warmup_; }
   value_type metropolis_hastings_rate() const { return (value_type)(mh_sum_ / std::max(iteration_ - burnin_, 1)); }
   void update(const value_type& mh_rate) {  // HMCParameters.h:259-297
     if (iteration_ == burnin_) {

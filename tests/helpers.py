@@ -65,3 +65,42 @@ def random_state(o, rng, imag=True):
         mu = mu.real + 0j
         p = p.real + 0j
     return mu, p
+
+
+def dense_posterior(o, Yh, s):
+    """Exact posterior of Re(mu) for a single-resolution chain with the noise
+    precision s held fixed: precision A = C^-1 + s P^T P, mean A^-1 s P^T Yh,
+    with C^-1 applied column by column as mu -> F(B(mu) / (N lambda))
+    (the prior term of MultiResParameters.h:337-376).  Needs an empty Z."""
+    assert o.Z.size == 0
+    N = o.N
+    Cinv = np.zeros((N, N))
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1.0
+        Cinv[:, j] = mrp.F(mrp.B(e.reshape(o.shape)) / (N * o.lam)).real.reshape(-1)
+    A = Cinv.copy()
+    A[o.I, o.I] += s
+    Sig = np.linalg.inv(A)
+    m = Sig[:, o.I] @ (s * np.asarray(Yh))
+    C = np.linalg.inv(Cinv)
+    return m, Sig, C
+
+
+def mc_compare(X, mean, var, nb=30):
+    """Monte-Carlo check of draws X (iterations x voxels) against exact
+    marginal means and variances.  Batch-means standard errors over nb
+    batches give per-voxel z-scores of the mean and effective sample sizes;
+    the variance ratio's standard error is taken as sqrt(2 / ESS).
+    Returns (z, ess, var_ratio, var_z)."""
+    X = np.asarray(X)
+    K = X.shape[0] - X.shape[0] % nb
+    X = X[:K]
+    Bm = X.reshape(nb, -1, X.shape[1]).mean(axis=1)
+    mcse = Bm.std(axis=0, ddof=1) / np.sqrt(nb)
+    z = (X.mean(axis=0) - mean) / mcse
+    v = X.var(axis=0)
+    ess = v / mcse ** 2
+    ratio = v / var
+    var_z = (ratio - 1.0) / np.sqrt(2.0 / ess)
+    return z, ess, ratio, var_z

@@ -418,3 +418,66 @@ def test_gradient_long_z_embedding(gpu_available, precision):
     assert abs((r["log_prior0"] + r["log_lik0"]) - lp0) < etol * abs(lp0)
     assert abs(r["k1"] - k1) < etol * abs(k1)
     assert abs(r["e0"] - k0) < etol * abs(k0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_rational_quadratic_spectrum_and_gradient(gpu_available, precision):
+    """--rational-quadratic (kernels.inl:58-74, MultiResParameters.h:234-243):
+    c = tau^2 (1 + psi/nu b^2)^-nu on the torus base, lambda = F(c).  Device
+    eigenvalues (always float64) against the oracle to 1e-12 of max|lambda|,
+    the nonpositive count exactly, then gradient and a leapfrog trajectory."""
+    from dualres_amd import Chain
+    theta = (0.9, 0.4, 1.3)
+    p = small_single(dims=(12, 10, 8), theta=theta)
+    embed = (24, 20, 18)
+    o = mrp.MultiResParameters(1, theta, p["ijk"], p["q_hr"], rng=mrp.NumpyRng(3), covfun="rq", M=embed)
+    data = mrp.MultiResData(p["Yh"])
+    c = Chain(p["ijk"], theta, p["q_hr"], n_datasets=1, precision=precision, state_mode="compat", covfun="rq",
+              embed=embed, seed=3)
+    c.set_data(p["Yh"])
+    lam_o = o.lam.real
+    assert np.max(np.abs(c.lambda_() - lam_o)) <= 1e-12 * np.max(np.abs(lam_o))
+    assert c.n_nonpositive == o.Z.size
+    rng = np.random.default_rng(8)
+    mu, mom = random_state(o, rng)
+    mu.reshape(-1)[o.I] += p["Yh"]
+    o.mu = mu.copy()
+    o.sigma_sq_inv = [np.float64(1.3)]
+    c.set_state(mu, mom)
+    c.set_sigma_sq_inv([1.3])
+    g_o = o.compute_gradient(data, o.mu)
+    assert rel(c.gradient().real, g_o.real) < (1e-10 if precision == "fp64" else 2e-5)
+    o._compute_mass_matrix_eigen_values()
+    mu1, p1 = o.leapfrog(data, mu.copy(), mom.copy(), 0.06, 5)
+    c.leapfrog(0.06, 5)
+    mu_e, p_e = c.get_state()
+    tol = 1e-9 if precision == "fp64" else 2e-4
+    assert rel(mu_e.real, mu1.real) < tol and rel(p_e.real, p1.real) < tol
+    c.close()
+
+
+def test_kriging_pattern_flat_offset_wrap(gpu_available):
+    """A highres image with every voxel present, so neighbourhoods of rows
+    near the x, y and z faces run past the grid.  The reference checks only
+    the flat offset (kriging_matrix.h:263-266): a neighbour past the x face
+    lands at the other end of the previous or next row and is kept.  Device
+    pattern bit-exact against the oracle, with such wrapped entries present."""
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
+    from oracle import kriging
+    theta = (0.806, 0.132, 1.5)
+    dims, sd = (14, 12, 10), (7, 6, 5)
+    rng = np.random.default_rng(1)
+    hr = rng.standard_normal(dims[::-1]) + 3.0
+    std = rng.standard_normal(sd[::-1]) + 3.0
+    q_hr = problem.qform_axis(2.0, dims)
+    q_std = problem.qform_axis(4.0, sd, origin=[q_hr[a, 3] + 1.0 for a in range(3)])
+    rp, cols, vals, nrow, ncol, st = kriging.get_sparse_kriging_matrix_data(hr, std, q_hr, q_std, theta, 4.6)
+    rows, P, _ = kriging.kriging_pattern(hr, std, q_hr, q_std, theta, 4.6)
+    base = 2 * problem.nonzero_ijk(std)       # stdres voxel centres sit on highres voxel 2 ijk
+    wrapped = sum(int(np.any((base[i] + P[sel] < 0) | (base[i] + P[sel] >= np.array(dims)), axis=1).sum())
+                  for i, (_, sel, _) in enumerate(rows))
+    assert wrapped > 1000
+    W = kriging_build(hr, q_hr, std, q_std, theta, 4.6)
+    assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
+    assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))

@@ -202,3 +202,64 @@ def test_kriging_cli_usage_errors(tmp_path):
         assert r.returncode == 1 and "not a NIfTI file" in r.stderr
         r = subprocess.run([exe, str(img), "--bogus"], capture_output=True, text=True)
         assert r.returncode == 1 and "Unrecognized option" in r.stderr
+
+
+def _hmc_trace(rates, burnin=100, n_save=10, thin=1, L=10, starting_eps=0.1, mh_target=0.65, gamma=0.05, t0=10.0,
+               kappa=0.75):
+    """Runs the C++ HMCParameters<double> of the CLIs (host/sampler.hpp) over a
+    rate sequence; one row per update: eps, eps_bar, h_bar, eps_target, L,
+    eps_start_found, warmup, mh rate."""
+    lib = _host()
+    lib.drh_hmc_trace.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 5 + \
+        [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    lib.drh_hmc_trace.restype = C.c_int
+    r = np.ascontiguousarray(rates, dtype=np.float64)
+    out = np.zeros((len(r), 8))
+    rc = lib.drh_hmc_trace(burnin, n_save, thin, L, starting_eps, mh_target, gamma, t0, kappa, _p(r, C.c_double),
+                           len(r), _p(out, C.c_double))
+    assert rc == 0
+    return out
+
+
+def test_cpp_hmc_parameters_initial_search():
+    """HMCParameters.h:259-279 in the C++ class: the starting eps doubles while
+    the rate stays above 1/2 (A = +1), halves while it stays below (A = -1);
+    L = 1 until found; then target = log(10 eps).  Hand-worked values, the
+    same known answers the oracle's HMCParameters is held to."""
+    t = _hmc_trace([0.9, 0.7, 0.3], starting_eps=0.1)
+    assert t[:, 0] == pytest.approx([0.2, 0.4, 0.4])
+    assert list(t[:, 4]) == [1, 1, 10] and list(t[:, 5]) == [0, 0, 1]
+    assert t[2, 3] == pytest.approx(math.log(4.0))
+    t = _hmc_trace([0.2, 0.45, 0.6], starting_eps=0.8)
+    assert t[:, 0] == pytest.approx([0.4, 0.2, 0.2])
+    assert t[2, 3] == pytest.approx(math.log(2.0)) and t[2, 5] == 1
+
+
+def test_cpp_hmc_parameters_dual_averaging_known_answers():
+    """Dual averaging (HMCParameters.h:280-290; Hoffman & Gelman 2014 Alg. 5):
+    after the search ends at eps = 0.4 (iteration 2), iteration 3 at rate 0.5
+    gives h_bar = 0.15/13 and eps = 4 exp(-sqrt(3) h_bar / 0.05) =
+    2.682073873340143, eps_bar = eps^(3^-0.75) = 1.5415811815734066; rate 0.8
+    at iteration 4 returns h_bar to 0 and eps to exp(target) = 4."""
+    t = _hmc_trace([0.9, 0.7, 0.3, 0.5, 0.8])
+    assert t[3, 2] == pytest.approx(0.15 / 13, rel=1e-14)
+    assert t[3, 0] == pytest.approx(2.682073873340143, rel=1e-13)
+    assert t[3, 1] == pytest.approx(1.5415811815734066, rel=1e-13)
+    assert t[4, 2] == pytest.approx(0.0, abs=1e-16)
+    assert t[4, 0] == pytest.approx(4.0, rel=1e-14)
+    assert t[4, 1] == pytest.approx(4 ** (4 ** -0.75) * 1.5415811815734066 ** (1 - 4 ** -0.75), rel=1e-13)
+    # a rate at the target keeps eps at ten times the start
+    t = _hmc_trace([0.2, 0.55] + [0.65] * 20, burnin=50, n_save=5, starting_eps=0.3)
+    assert t[2:, 0] == pytest.approx(np.full(20, 1.5), rel=1e-12)
+
+
+def test_cpp_hmc_parameters_burnin_end_and_floor():
+    """At iteration == burnin the sampler adopts eps_bar and starts counting MH
+    rates (HMCParameters.h:259-266, 292-296); during dual averaging eps never
+    falls below 1e-5 (HMCParameters.h:287)."""
+    rates = [0.9, 0.7, 0.3, 0.5, 0.8, 0.65, 0.7, 0.5]
+    t = _hmc_trace(rates, burnin=6, n_save=3, thin=2)
+    assert t[6, 0] == t[5, 1] and t[6, 6] == 0 and t[5, 6] == 1
+    assert t[6, 7] == pytest.approx(0.7) and t[7, 7] == pytest.approx(0.6)
+    t = _hmc_trace([0.3, 0.6] + [0.0] * 60, burnin=1000, n_save=1)
+    assert np.all(t[2:, 0] >= 1e-5) and t[-1, 0] == 1e-5

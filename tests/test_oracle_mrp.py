@@ -135,11 +135,117 @@ def test_update_accept_and_energies_consistent():
     rng = np.random.default_rng(4)
     zr, zi = rng.standard_normal(o.N), rng.standard_normal(o.N)
     o.rng = mrp.ScriptedRng(normals=[(zr, zi)], uniforms=[0.0], gammas=[2.0, 3.0])
+    mu_start = o.mu.copy()
+    lp_start = o._log_posterior(data, mu_start)
     mh = o.update(data, 0.02, 3)
     assert 0.0 <= mh <= 1.0
-    # E0 = -1/2 sum |z|^2 is the kinetic energy of the freshly drawn momentum
     assert o.accepted  # u = 0 < R always unless R = 0
     assert o.sigma_sq_inv == [2.0, 3.0]
+    # H0 = E0 + log posterior of the start, with E0 = -1/2 sum |z|^2
+    e0 = -0.5 * sum(a * a + b * b for a, b in zip(zr.tolist(), zi.tolist()))
+    assert o.initial_energy == pytest.approx(e0 + lp_start, rel=1e-12)
+
+
+def _neg(o, f):
+    """Flat index of the mode -k for the flat index f of k."""
+    M0, M1, M2 = o.M
+    x, y, z = f % M0, (f // M0) % M1, f // (M0 * M1)
+    return (-x) % M0 + M0 * ((-y) % M1 + M1 * ((-z) % M2))
+
+
+def _fresh_momentum_kinetic(o, zr, zi):
+    o._compute_mass_matrix_eigen_values()
+    o.rng = mrp.ScriptedRng(normals=[(zr, zi)])
+    e0 = o._sample_momentum()
+    return e0, o._potential_energy(o.momentum)
+
+
+def test_sample_momentum_kinetic_energy_equals_e0():
+    """_sample_momentum (MultiResParameters.h:783-815): p = F(sqrt(lambda_m)
+    B(z) / N).  Since B F = N I, B(p) = sqrt(lambda_m) B(z), so the kinetic
+    energy of the fresh momentum is -1/2 sum_{k not in Z} |B(z)_k|^2 / N, which
+    by Parseval is E0 = -1/2 sum |z|^2 when Z is empty.  A dropped square
+    root or a lost 1/N breaks the equality."""
+    o, _ = _tiny(True, seed=11)
+    assert o.Z.size == 0
+    rng = np.random.default_rng(12)
+    zr, zi = rng.standard_normal(o.N), rng.standard_normal(o.N)
+    e0, k = _fresh_momentum_kinetic(o, zr, zi)
+    assert e0 == pytest.approx(-0.5 * float(np.sum(zr ** 2 + zi ** 2)), rel=1e-14)
+    assert k == pytest.approx(e0, rel=1e-12)
+    # a non-empty low-rank set: the kinetic energy loses exactly the energy of
+    # B(z) on Z (Z is closed under k -> -k because lambda is even)
+    o.Z = np.array([1, _neg(o, 1), 5, _neg(o, 5)])
+    lam = o.lam.reshape(-1).copy()
+    assert np.allclose(lam[[1, 5]], lam[[_neg(o, 1), _neg(o, 5)]])
+    e0, k = _fresh_momentum_kinetic(o, zr, zi)
+    bz = mrp.B((zr + 1j * zi).reshape(o.shape)).reshape(-1)
+    keep = np.ones(o.N, bool)
+    keep[o.Z] = False
+    assert k == pytest.approx(-0.5 * float(np.sum(np.abs(bz[keep]) ** 2)) / o.N, rel=1e-12)
+    assert k > e0
+
+
+def test_initialize_mu_is_a_tenth_scaled_gp_draw():
+    """_initialize_mu (MultiResParameters.h:743-759): mu0 = B(F(z) sqrt(lambda)
+    / (10 N)) with Z zeroed.  Its prior energy is then -1/2 sum_{k not in Z}
+    |F(z)_k|^2 / (100 N) (B B = N x reflection and lambda is even), i.e.
+    -1/2 sum |z|^2 / 100 when Z is empty: a GP draw scaled by 1/10."""
+    o, _ = _tiny(False, seed=13)
+    rng = np.random.default_rng(14)
+    zr, zi = rng.standard_normal(o.N), rng.standard_normal(o.N)
+    o.rng = mrp.ScriptedRng(normals=[(zr, zi)])
+    o._initialize_mu()
+    assert o.log_prior() == pytest.approx(-0.5 * float(np.sum(zr ** 2 + zi ** 2)) / 100.0, rel=1e-12)
+    # with k = 5 the scale is 1/25; with a non-empty Z the energy on Z is dropped
+    o.Z = np.array([2, _neg(o, 2)])
+    o.rng = mrp.ScriptedRng(normals=[(zr, zi)])
+    o._initialize_mu(k=5)
+    fz = mrp.F((zr + 1j * zi).reshape(o.shape)).reshape(-1)
+    keep = np.ones(o.N, bool)
+    keep[o.Z] = False
+    lp = -0.5 * float(np.sum(np.abs(fz[keep]) ** 2)) / (25.0 * o.N)
+    assert o.log_prior() == pytest.approx(lp, rel=1e-12)
+    assert np.allclose(o.mu.reshape(-1)[o.I].real, o.real_mu)
+
+
+def test_sigma_gibbs_shape_and_rate():
+    """_update_sigma_sq_inv (MultiResParameters.h:820-859): Gamma(n_h/2 + 1,
+    scale 1 / (||Yh - Re mu[I]||^2 / 2)) then Gamma(n_s/2 + 1, scale
+    1 / (||Ys - W Re mu[I]||^2 / 2)); the rates are recomputed here with loops
+    over the CSR rows.  _initialize_sigma (:764-778): Gamma(tau^2, 1), then
+    sigma_s = sigma_h / U."""
+    o, data = _tiny(True, seed=15)
+    calls = []
+
+    class Capture:
+        def gamma(self, shape, scale):
+            calls.append((shape, scale))
+            return 1.0 + len(calls)
+
+        def uniform(self):
+            return 0.25
+
+    o.rng = Capture()
+    mu_re = [o.mu.reshape(-1)[i].real for i in o.I]
+    o.real_mu = o._gather_re(o.mu)
+    o._update_sigma_sq_inv(data)
+    ssh = sum((y - m) ** 2 for y, m in zip(data.Yh.tolist(), mu_re))
+    W = data.W
+    sss = 0.0
+    for r in range(W.shape[0]):
+        acc = 0.0
+        for t in range(W.indptr[r], W.indptr[r + 1]):
+            acc += W.data[t] * mu_re[W.indices[t]]
+        sss += (data.Ys[r] - acc) ** 2
+    assert calls[0][0] == 0.5 * len(o.I) + 1 and calls[1][0] == 0.5 * W.shape[0] + 1
+    assert calls[0][1] == pytest.approx(1.0 / (0.5 * ssh), rel=1e-12)
+    assert calls[1][1] == pytest.approx(1.0 / (0.5 * sss), rel=1e-12)
+    assert o.sigma_sq_inv == [2.0, 3.0]
+    calls.clear()
+    o._initialize_sigma(THETA[0])
+    assert calls == [(THETA[0], 1.0)]
+    assert o.sigma_sq_inv == [2.0, 2.0 / 0.25]
 
 
 def test_tau_update_matches_rescaled_covariance():
