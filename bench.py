@@ -267,26 +267,91 @@ def batch_leg(n_chains=32, workers=8, seed=7):
     return out
 
 
-def cpu_baseline(p, workers):
-    """Complex-state oracle (reference structure: c2c FFTs, separate W and W^T)."""
-    import scipy.fft as sfft
-    from oracle import mrp
+def host_cores():
+    """What the host offers this job: logical CPUs, the CPUs this process may
+    run on, physical cores (distinct physical id / core id pairs), and the
+    thread count the CPU baseline uses (the CPUs this process may run on,
+    at most 16: one GPU's share of the host)."""
+    n_logical = os.cpu_count() or 1
+    try:
+        n_affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n_affinity = n_logical
+    phys = set()
+    try:
+        pid = cid = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":")[1].strip()
+                elif not line.strip() and pid is not None:
+                    phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    threads = max(1, min(16, n_affinity))
+    return {"nproc": n_logical, "affinity": n_affinity, "physical_cores": len(phys) or None, "threads": threads,
+            "threads_80pct": max(1, threads * 4 // 5)}
+
+
+def cpu_iterations(p, precision, threads, w_threads, iters, eps, tag):
+    """Seconds of each of `iters` HMC iterations (L = 25, momentum, energies,
+    accept, Gibbs draws) of the C++ restatement of the reference's sampler
+    (oracle/cpu_hmc.cpp) at the benchmark workload."""
+    from oracle import cpu_port
     t = time.perf_counter()
-    o = mrp.MultiResParameters(2, THETA, p["ijk"], p["q_hr"], rng=mrp.NumpyRng(3), dtype=np.float32, M=EMBED)
-    data = mrp.MultiResData(p["Yh"], p["Ys"], p["W"], dtype=np.float32)
-    o.add_data_to_mu(data)
-    o.sigma_sq_inv = [np.float32(1.0), np.float32(4.0)]
-    o._compute_mass_matrix_eigen_values()
-    log(f"[bench] cpu oracle setup {time.perf_counter() - t:.1f}s")
-    steps = 4
-    with sfft.set_workers(workers):
+    c = cpu_port.CpuChain(p["ijk"], THETA, p["q_hr"], EMBED, p["Yh"], p["Ys"], p["W"], precision=precision,
+                          threads=threads, w_threads=w_threads, seed=42)
+    c.add_data_to_mu()
+    c.set_sigma_sq_inv([1.0, 4.0])
+    log(f"[bench] cpu {tag}: set-up {time.perf_counter() - t:.1f}s")
+    out = []
+    for i in range(iters):
         t = time.perf_counter()
-        o.leapfrog(data, o.mu.copy(), np.zeros_like(o.mu), 1e-3, steps)
-        dt = time.perf_counter() - t
-    return {"value": (steps + 1) / dt, "unit": "grad evals/s", "cores": workers, "kind": "port",
-            "sample": f"{steps}-step leapfrog ({steps + 1} gradient evaluations, {steps} position updates) of the "
-                      f"complex-state oracle (c2c pocketfft with {workers} workers, scipy.sparse W and W^T single-"
-                      f"threaded) at the same workload, float32, {dt:.1f}s"}
+        c.update(eps, L_STEPS)
+        out.append(time.perf_counter() - t)
+        log(f"[bench] cpu {tag}: iteration {i + 1}/{iters} {out[-1]:.1f}s")
+    c.close()
+    return out
+
+
+CPU_SAMPLE = ("post-burn-in HMC iterations (L = 25: momentum draw, 26 gradient evaluations, energies, accept, "
+              "Gibbs draws) of the C++17/OpenMP restatement of the reference's sampler (complex state, 4 complex 3D "
+              "DFTs per leapfrog step with its own radix-4/2/3/5/7 DFT, W and W^T products on one thread) at the "
+              "same workload")
+
+
+def cpu_baseline(p, eps):
+    """One post-burn-in iteration of the C++ CPU restatement in float32 on this
+    job's share of the host cores, plus the committed study (float32 and
+    float64, all and 80% of the threads, OpenMP W, medians of 3 iterations)."""
+    hc = host_cores()
+    t = cpu_iterations(p, "fp32", hc["threads"], 1, 1, eps, "fp32")[0]
+    out = {"value": (L_STEPS + 1) / t, "unit": "grad evals/s", "cores": hc["threads"], "kind": "port",
+           "sample": f"1 of the {CPU_SAMPLE}, float32, {hc['threads']} threads, {t:.1f}s",
+           "host": hc}
+    path = os.path.join(ROOT, "profiles", "cpu_baseline_dualres_1mm.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            out["study"] = json.load(f)
+    return out
+
+
+def cpu_study(p, eps, iters=3):
+    """Medians of `iters` iterations: float32 and float64, all threads and 80%
+    of them (the reference's default share, defines.h:73-74) with W on one
+    thread, and all threads with OpenMP W."""
+    hc = host_cores()
+    runs = []
+    for prec in ("fp32", "fp64"):
+        for threads, w in ((hc["threads"], 1), (hc["threads_80pct"], 1), (hc["threads"], hc["threads"])):
+            ts = cpu_iterations(p, prec, threads, w, iters, eps, f"{prec} {threads} threads, W on {w}")
+            med = float(np.median(ts))
+            runs.append({"precision": prec, "threads": threads, "w_threads": w, "iteration_s": [round(x, 2) for x in ts],
+                         "median_s": round(med, 2), "grad_evals_s": round((L_STEPS + 1) / med, 3)})
+    return {"host": hc, "sample": f"median of {iters} {CPU_SAMPLE}", "eps": eps, "runs": runs}
 
 
 def main():
@@ -303,6 +368,9 @@ def main():
                     help="also run a plain 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the --precision complex-state chain (for profiler runs)")
+    ap.add_argument("--cpu-study", action="store_true",
+                    help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
+                         "medians of 3 iterations) and print its JSON")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving leg and print its JSON (latency study with --serve-requests)")
     a = ap.parse_args()
@@ -315,6 +383,9 @@ def main():
                                      n_req=a.serve_requests)), flush=True)
         return
     p = build_problem()
+    if a.cpu_study:
+        print(json.dumps(cpu_study(p, eps=0.54)), flush=True)
+        return
     results = {}
     main_chain = None
     for prec in ((a.precision,) if a.main_only else ("fp32", "fp64")):
@@ -353,7 +424,7 @@ def main():
     batch = None if a.main_only or a.no_serve else batch_leg()
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(p, workers=min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(p, eps=results[a.precision]["eps"])
         log(f"[bench] cpu baseline {cpu['value']:.3f} grad evals/s")
     r = results[a.precision]
     line = {

@@ -15,6 +15,7 @@ DR_OK, DR_EDOMAIN, DR_ELOGIC, DR_ERUNTIME, DR_EHIP, DR_ENOMEM, DR_EBUSY = range(
 DR_FP64, DR_FP32 = 0, 1
 DR_STATE_COMPAT, DR_STATE_REAL = 0, 1
 DR_COV_RBF, DR_COV_RQ = 0, 1
+DR_BATCH_GRAPH, DR_BATCH_STREAMS, DR_BATCH_EAGER = 0, 1, 2
 
 
 class DrError(RuntimeError):
@@ -87,6 +88,10 @@ SIGNATURES = {
     "dr_mu_data_sumsq": [_P, _D],
     "dr_update": [_P, C.c_double, C.c_int, C.c_double, C.POINTER(dr_update_result)],
     "dr_update_with_noise": [_P, C.c_double, C.c_int, C.c_double, _D, _D, C.POINTER(dr_update_result)],
+    "dr_update_captured": [_P, C.c_double, C.c_int, C.c_double, C.POINTER(dr_update_result)],
+    "dr_batch_create": [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P)],
+    "dr_batch_update": [_P, _D, C.POINTER(C.c_int), _D, C.POINTER(dr_update_result)],
+    "dr_batch_destroy": [_P],
     "dr_log_prior": [_P, _D],
     "dr_log_likelihood": [_P, _D],
     "dr_log_posterior": [_P, _D],

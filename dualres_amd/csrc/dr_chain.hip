@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -176,6 +177,15 @@ struct ChainBase {
   virtual void get_lambda(double* out) = 0;
   virtual void reset_kernel_profile() = 0;
   virtual void kernel_profile(double* ms, int max_k, int* n_k, int64_t* n_steps) = 0;
+  // captured iterations (one HIP graph per leapfrog length; see Chain)
+  uint64_t layout_gen = 0;  // changes whenever the buffers a captured graph uses move
+  virtual void update_graph(double eps, int L, double u, dr_update_result* res) = 0;
+  virtual void set_batch_slots(void* dev_scalars, void* host_scalars, double* dev_sums) = 0;
+  virtual void stage_scalars(double eps) = 0;
+  virtual hipGraph_t capture_iteration(int L) = 0;
+  virtual void enqueue_iteration(double eps, int L, hipStream_t s) = 0;
+  virtual void enqueue_eager(double eps, int L) = 0;
+  virtual void finish_iteration(const double* h, double u, dr_update_result* res, bool copy, hipStream_t s) = 0;
 
   long region_index(int x, int y, int z) const { return x + (long)reg[0] * (y + (long)reg[1] * z); }
   double log_lik(double sh, double ss) const {
@@ -225,6 +235,18 @@ struct Chain : public ChainBase {
   int slot_nblk[NSLOT] = {0};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // per-kernel profile of one leapfrog step: events between launches
+  // Captured iterations.  The kernels of a captured HMC iteration read their
+  // per-iteration scalars from dsc (staged in pinned host memory hsc and
+  // copied before each launch), and finalize writes the energy sums to
+  // sums_out; a batch points both at its own arrays.  gsc is set while
+  // capturing, so the functors built then bind to dsc.
+  DevBuf<StepScalars<T>> sc_own;
+  StepScalars<T>* dsc = nullptr;
+  StepScalars<T>* hsc = nullptr;
+  bool hsc_own = false;
+  const StepScalars<T>* gsc = nullptr;
+  double* sums_out = nullptr;
+  std::map<int, hipGraphExec_t> gexec;
   static constexpr int kMarks = 12;
   hipEvent_t pev[kMarks] = {};
   int rec = -1, nrec = 0;
@@ -238,6 +260,8 @@ struct Chain : public ChainBase {
 
   ~Chain() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    if (hsc_own && hsc) (void)hipHostFree(hsc);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : pev)
@@ -461,7 +485,7 @@ struct Chain : public ChainBase {
     if (zinj) {
       xr2c<T>(M[1] * M[2], LoadNoiseBuf<T>{zinj, M[0], pslot(slot), 0.0}, StoreRowC<T>{{}, T2.p, H0}, slot);
     } else {
-      xr2c<T>(M[1] * M[2], LoadNoise<T>{cfg.seed, stream_id, M[0] / 2, pslot(slot), 0.0},
+      xr2c<T>(M[1] * M[2], LoadNoise<T>{cfg.seed, stream_id, M[0] / 2, pslot(slot), 0.0, gsc, (int)(stream_id & 1)},
               StoreRowC<T>{{}, T2.p, H0}, slot);
     }
     strided<T, false>(1, M[2], H0, M[1], M[1], LoadS<T>{{}, T2.p, (long)M[1] * H0, H0},
@@ -474,11 +498,11 @@ struct Chain : public ChainBase {
     a.nslot = NSLOT;
     for (int s = 0; s < NSLOT; ++s) a.nblk[s] = slot_nblk[s];
     a.pstride = pstride;
-    hipLaunchKernelGGL(k_finalize, dim3(NSLOT), dim3(kThreads), 0, stream, a, part.p, sums.p);
+    hipLaunchKernelGGL(k_finalize, dim3(NSLOT), dim3(kThreads), 0, stream, a, part.p, sums_out ? sums_out : sums.p);
     DR_HIP(hipGetLastError());
   }
   void read_sums(double* h) {
-    DR_HIP(hipMemcpyAsync(h, sums.p, NSLOT * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DR_HIP(hipMemcpyAsync(h, sums_out ? sums_out : sums.p, NSLOT * sizeof(double), hipMemcpyDeviceToHost, stream));
     DR_HIP(hipStreamSynchronize(stream));
   }
   void clear_slots() {
@@ -495,6 +519,7 @@ struct Chain : public ChainBase {
 
   // ------------------------------------------------------------ region / data
   void set_region(const int r[3]) override {
+    drop_graphs();
     for (int a = 0; a < 3; ++a)
       DR_REQUIRE(r[a] >= 1 && r[a] <= M[a], DR_EDOMAIN, "image region exceeds the embedding torus");
     if (r[0] == reg[0] && r[1] == reg[1] && r[2] == reg[2]) return;
@@ -1139,6 +1164,7 @@ struct Chain : public ChainBase {
   }
 
   void upload_data() override {
+    drop_graphs();
     // data on the region grid
     std::vector<T> yg(nreg, (T)0);
     std::vector<uint8_t> mk_(nreg, 0);
@@ -1253,7 +1279,7 @@ struct Chain : public ChainBase {
         const WmuBnd<T> ab{ws.n_bnd, ws.brow.p, ws.brp.p, ws.bbase.p, ws.bd.p, ws.bw.p, R0, R01, ws.Vc.p};
         const WmuFar<T> af{ws.n_far, ws.frow.p, ws.frp.p, ws.fcol.p, ws.fw.p, ws.Vf.p};
         hipLaunchKernelGGL(k_wmu_sb<T>, dim3(gi + gb + gf), dim3(kThreads), (size_t)ws.ncw * (4 + 4 * sizeof(T)), stream,
-                           ai, ab, af, gi, gb, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr);
+                           ai, ab, af, gi, gb, R, Ysd.p, (T)sigma[1], slot >= 0 ? pslot(slot) : nullptr, gsc);
         DR_HIP(hipGetLastError());
       }
       mark();
@@ -1287,7 +1313,7 @@ struct Chain : public ChainBase {
     }
     if (slot >= 0) note_blocks(slot, g1);
     hipLaunchKernelGGL(k_wmu<T>, dim3(g1), dim3(kThreads), 0, stream, (int)n_s, Wrp.p, Wcols.p, Wvals.p, R, Ysd.p,
-                       (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr);
+                       (T)sigma[1], vsd.p, slot >= 0 ? pslot(slot) : nullptr, gsc);
     DR_HIP(hipGetLastError());
     mark();
     hipLaunchKernelGGL(k_wtv<T>, dim3(g2), dim3(kThreads), 0, stream, (int)n_h, WTrp.p, WTrows.p, WTvals.p, vsd.p,
@@ -1298,7 +1324,7 @@ struct Chain : public ChainBase {
   LoadResidual<T> residual(const T* R, int slot) {
     return LoadResidual<T>{R, Yg.p, mask.p, cfg.n_datasets == 2 ? WTv.p : nullptr,
                            cfg.n_datasets == 2 && use_cls ? WTvb.p : nullptr, (T)sigma[0], reg[0],
-                           slot >= 0 ? pslot(slot) : nullptr, 0.0};
+                           slot >= 0 ? pslot(slot) : nullptr, 0.0, gsc};
   }
   // inverse y and x passes from T1 (z < n2) to the real region field R
   void c2r_yx(T* R) {
@@ -1367,7 +1393,7 @@ struct Chain : public ChainBase {
         mark();
       }
       MidLeapfrog<T> mid{s == 0 ? muA.p : muB.p, muB.p, p.p, invlam.p, sk, (T)((s == 0 ? 0.5 : 1.0) * eps),
-                         (T)eps, (T)sigma[0], H0, s == 0 ? 1 : 0, pslot(S_PRIOR_A), 0.0};
+                         (T)eps, (T)sigma[0], H0, s == 0 ? 1 : 0, pslot(S_PRIOR_A), 0.0, gsc, s == 0 ? 0 : 1};
       zfused(mid, s == 0 ? S_PRIOR_A : -1);
       c2r_yx(RB.p);
       w_products(RB.p, last ? S_SSES_B : -1);
@@ -1380,7 +1406,7 @@ struct Chain : public ChainBase {
     }
     // final half kick at B with the energies of the end point
     EpiMomentum<T> em{muB.p, p.p, invlam.p, sk, (T)(0.5 * eps), (T)sigma[0], H0, nullptr, nullptr, 1, part.p,
-                      pstride, S_PRIOR_B, S_TMP, S_KIN_B, S_TMP, 0.0, 0.0, 0.0, 0.0};
+                      pstride, S_PRIOR_B, S_TMP, S_KIN_B, S_TMP, 0.0, 0.0, 0.0, 0.0, gsc};
     const int zb = zgrid();
     note_blocks(S_PRIOR_B, zb);
     note_blocks(S_KIN_B, zb);
@@ -1394,35 +1420,40 @@ struct Chain : public ChainBase {
       note_blocks(S_KINI_B, g);
       hipLaunchKernelGGL(k_imag_leapfrog<T>, dim3(g), dim3(kThreads), 0, stream, Nh, H0, miA.p, miB.p, pim.p,
                          invlam.p, (T)eps, (T)(0.5 * eps), (T)eps, (T)sigma[0], L, pslot(S_PRIORI_A),
-                         pslot(S_PRIORI_B), pslot(S_KINI_B));
+                         pslot(S_PRIORI_B), pslot(S_KINI_B), gsc);
       DR_HIP(hipGetLastError());
     }
   }
 
   void accept_swap() {
+    drop_graphs();  // captured graphs hold the old buffer roles
     std::swap(muA.p, muB.p);
     std::swap(RA.p, RB.p);
     if (compat()) std::swap(miA.p, miB.p);
   }
 
-  void update(double eps, int L, double u, const double* zr, const double* zi, dr_update_result* res) override {
-    DR_REQUIRE(data_set, DR_ELOGIC, "update: data not set");
-    DR_REQUIRE(L >= 1, DR_EDOMAIN, "update: L must be >= 1");
-    clear_slots();
+  // momentum draw and leapfrog of one HMC iteration, then the energy sums
+  // (MultiResParameters.h:680-705); normals injected when zr is given
+  void device_iteration(double eps, int L, uint64_t it, const double* zr, const double* zi) {
     const long sk = (long)M[1] * H0;
-    // momentum p = sqrt(lambda_m) F(z)
-    const uint64_t it = iteration++;
+    clear_slots();
     if (zr) upload_real(zr, zbuf);
-    noise_r2c(4 * it + 0, zr ? zbuf.p : nullptr, S_E0, EpiNoiseMomentum<T>{{}, p.p, invlam.p, sk, (T)sigma[0]});
+    noise_r2c(4 * it + 0, zr ? zbuf.p : nullptr, S_E0,
+              EpiNoiseMomentum<T>{{}, p.p, invlam.p, sk, (T)sigma[0], gsc});
     if (compat()) {
       if (zr) upload_real(zi, zbuf);
-      noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I, EpiNoiseMomentum<T>{{}, pim.p, invlam.p, sk, (T)sigma[0]});
+      noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I,
+                EpiNoiseMomentum<T>{{}, pim.p, invlam.p, sk, (T)sigma[0], gsc});
     }
     leapfrog_core(eps, L);
     finalize();
-    double h[NSLOT];
-    read_sums(h);
-    if (prof && L >= 2) accumulate_profile(L - 1);
+  }
+
+  // energies, accept test and the kept state's statistics from the sums h
+  // (:701-727).  On accept the proposal becomes state A by a pointer swap, or
+  // for a captured iteration (whose graph always reads A and writes B) by
+  // copies enqueued on s.
+  void finish_iteration(const double* h, double u, dr_update_result* res, bool copy, hipStream_t s) override {
     const double nN = (double)N;
     const double e0 = -0.5 * (h[S_E0] + (compat() ? h[S_E0I] : 0.0));
     const double lp0 = -0.5 * (h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0)) / nN;
@@ -1436,7 +1467,8 @@ struct Chain : public ChainBase {
     if (std::isnan(R)) R = 0.0;
     const bool acc = u < R;
     if (acc) {
-      accept_swap();
+      if (copy) accept_copy(s);
+      else accept_swap();
       prior_sum = h[S_PRIOR_B] + (compat() ? h[S_PRIORI_B] : 0.0);
       sse_h = h[S_SSEH_B];
       sse_s = h[S_SSES_B];
@@ -1461,6 +1493,124 @@ struct Chain : public ChainBase {
       res->sse_s = sse_s;
       res->accepted = acc ? 1 : 0;
     }
+  }
+
+  void update(double eps, int L, double u, const double* zr, const double* zi, dr_update_result* res) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "update: data not set");
+    DR_REQUIRE(L >= 1, DR_EDOMAIN, "update: L must be >= 1");
+    const uint64_t it = iteration++;
+    device_iteration(eps, L, it, zr, zi);
+    double h[NSLOT];
+    read_sums(h);
+    if (prof && L >= 2) accumulate_profile(L - 1);
+    finish_iteration(h, u, res, false, stream);
+  }
+
+  // ------------------------------------------------------------ captured iterations
+  void accept_copy(hipStream_t s) {
+    DR_HIP(hipMemcpyAsync(muA.p, muB.p, Nh * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
+    if (compat()) DR_HIP(hipMemcpyAsync(miA.p, miB.p, Nh * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
+    DR_HIP(hipMemcpyAsync(RA.p, RB.p, (nreg + 4) * sizeof(T), hipMemcpyDeviceToDevice, s));
+  }
+  // forget captured graphs when buffers move; a batch compares layout_gen
+  void drop_graphs() {
+    for (auto& kv : gexec) (void)hipGraphExecDestroy(kv.second);
+    gexec.clear();
+    ++layout_gen;
+  }
+  void ensure_scalars() {
+    if (!dsc) {
+      sc_own.alloc(1);
+      dsc = sc_own.p;
+    }
+    if (!hsc) {
+      void* h = nullptr;
+      DR_HIP(hipHostMalloc(&h, sizeof(StepScalars<T>), hipHostMallocDefault));
+      hsc = static_cast<StepScalars<T>*>(h);
+      hsc_own = true;
+    }
+  }
+  void set_batch_slots(void* dev_scalars, void* host_scalars, double* dev_sums) override {
+    DR_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
+    if (hsc_own && hsc) (void)hipHostFree(hsc);
+    hsc_own = false;
+    hsc = static_cast<StepScalars<T>*>(host_scalars);
+    dsc = static_cast<StepScalars<T>*>(dev_scalars);
+    sums_out = dev_sums;
+    if (!dsc) sc_own.release();
+  }
+  // the next iteration's scalars in the host staging (the eager update's
+  // conversions to T and Philox streams 4 it, 4 it + 1)
+  void stage_scalars(double eps) override {
+    ensure_scalars();
+    const uint64_t it = iteration++;
+    hsc->half = (T)(0.5 * eps);
+    hsc->full = (T)eps;
+    hsc->sig0 = (T)sigma[0];
+    hsc->sig1 = (T)sigma[1];
+    hsc->stream[0] = 4 * it + 0;
+    hsc->stream[1] = 4 * it + 1;
+  }
+  // the device work of one iteration with the leapfrog length L as a graph
+  hipGraph_t capture_iteration(int L) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "update: data not set");
+    DR_REQUIRE(L >= 1, DR_EDOMAIN, "update: L must be >= 1");
+    ensure_scalars();
+    const bool prof0 = prof;
+    prof = false;
+    rec = -1;
+    hipGraph_t g = nullptr;
+    DR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      gsc = dsc;
+      device_iteration(0.0, L, 0, nullptr, nullptr);
+      gsc = nullptr;
+    } catch (...) {
+      gsc = nullptr;
+      prof = prof0;
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    prof = prof0;
+    DR_HIP(hipStreamEndCapture(stream, &g));
+    return g;
+  }
+  hipGraphExec_t graph_for(int L) {
+    auto f = gexec.find(L);
+    if (f != gexec.end()) return f->second;
+    hipGraph_t g = capture_iteration(L);
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    DR_HIP(err);
+    gexec[L] = e;
+    return e;
+  }
+  // stage the scalars, copy them, launch the graph for L and copy the sums
+  // back to the host staging after it, all on s
+  void enqueue_iteration(double eps, int L, hipStream_t s) override {
+    hipGraphExec_t e = graph_for(L);
+    stage_scalars(eps);
+    DR_HIP(hipMemcpyAsync(dsc, hsc, sizeof(StepScalars<T>), hipMemcpyHostToDevice, s));
+    DR_HIP(hipGraphLaunch(e, s));
+  }
+  // the eager launches of one iteration (the update's kernels, on the
+  // chain's stream, no wait)
+  void enqueue_eager(double eps, int L) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "update: data not set");
+    DR_REQUIRE(L >= 1, DR_EDOMAIN, "update: L must be >= 1");
+    const uint64_t it = iteration++;
+    device_iteration(eps, L, it, nullptr, nullptr);
+  }
+  void update_graph(double eps, int L, double u, dr_update_result* res) override {
+    enqueue_iteration(eps, L, stream);
+    double h[NSLOT];
+    read_sums(h);
+    finish_iteration(h, u, res, true, stream);
+    DR_HIP(hipStreamSynchronize(stream));
   }
 
   void leapfrog(double eps, int L, dr_update_result* res) override {
@@ -1876,6 +2026,13 @@ int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept, const 
   });
 }
 
+int dr_update_captured(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res) {
+  return guarded([&] {
+    DR_REQUIRE(c, DR_ELOGIC, "null chain");
+    c->impl->update_graph(eps, L, u_accept, res);
+  });
+}
+
 int dr_log_prior(dr_chain* c, double* out) {
   return guarded([&] {
     auto& k = *c->impl;
@@ -1919,6 +2076,159 @@ int dr_gradient(dr_chain* c, double* g_re, double* g_im) {
 }
 int dr_leapfrog(dr_chain* c, double eps, int L, dr_update_result* res) {
   return guarded([&] { c->impl->leapfrog(eps, L, res); });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Many chains advanced together, one HMC iteration per dr_batch_update.  In
+// DR_BATCH_GRAPH mode every chain's iteration (momentum draw, L leapfrog
+// steps, energy sums) is captured as its own graph and the graphs become
+// parallel branches of one graph, launched once per iteration on the batch
+// stream after one copy of all chains' scalars; one copy returns all energy
+// sums.  The graphs depend on each chain's L, so a new L vector captures and
+// instantiates anew.  DR_BATCH_STREAMS launches each chain's own graph on
+// its own stream, DR_BATCH_EAGER the plain kernel launches.  A chain's
+// kernels, scalars and summation order are the same in every mode and alone,
+// so its results do not depend on the other chains.
+struct dr_batch {
+  std::vector<dr_chain*> chains;
+  int mode = DR_BATCH_GRAPH;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* dsc = nullptr;
+  void* hsc = nullptr;
+  double* dsums = nullptr;
+  double* hsums = nullptr;
+  std::vector<int> cur_L;
+  std::vector<uint64_t> cur_gen;
+  hipGraphExec_t exec = nullptr;
+  static constexpr size_t kSlot = 64;  // bytes of one chain's scalars (StepScalars<double> = 48)
+
+  ~dr_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (dr_chain* c : chains) {
+      try {
+        c->impl->set_batch_slots(nullptr, nullptr, nullptr);
+      } catch (...) {
+      }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (dsc) (void)hipFree(dsc);
+    if (dsums) (void)hipFree(dsums);
+    if (hsc) (void)hipHostFree(hsc);
+    if (hsums) (void)hipHostFree(hsums);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  double* host_sums(size_t i) { return hsums + drh::NSLOT * i; }
+
+  void rebuild(const int* L) {
+    if (exec) {
+      (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+    }
+    hipGraph_t parent = nullptr;
+    DR_HIP(hipGraphCreate(&parent, 0));
+    try {
+      for (size_t i = 0; i < chains.size(); ++i) {
+        hipGraph_t g = chains[i]->impl->capture_iteration(L[i]);
+        hipGraphNode_t node;
+        const hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, g);
+        (void)hipGraphDestroy(g);
+        DR_HIP(e);
+      }
+      DR_HIP(hipGraphInstantiate(&exec, parent, nullptr, nullptr, 0));
+    } catch (...) {
+      (void)hipGraphDestroy(parent);
+      throw;
+    }
+    (void)hipGraphDestroy(parent);
+    cur_L.assign(L, L + chains.size());
+    cur_gen.clear();
+    for (dr_chain* c : chains) cur_gen.push_back(c->impl->layout_gen);
+  }
+  bool stale(const int* L) const {
+    if (!exec || !std::equal(cur_L.begin(), cur_L.end(), L)) return true;
+    for (size_t i = 0; i < chains.size(); ++i)
+      if (chains[i]->impl->layout_gen != cur_gen[i]) return true;
+    return false;
+  }
+
+  void update(const double* eps, const int* L, const double* u, dr_update_result* res) {
+    const size_t n = chains.size();
+    for (size_t i = 0; i < n; ++i) {
+      DR_REQUIRE(L[i] >= 1, DR_EDOMAIN, "update: L must be >= 1");
+      DR_REQUIRE(chains[i]->impl->data_set, DR_ELOGIC, "update: data not set");
+      DR_HIP(hipStreamSynchronize(chains[i]->impl->stream));  // earlier work on the chain's own stream
+    }
+    if (mode == DR_BATCH_GRAPH) {
+      if (stale(L)) rebuild(L);
+      for (size_t i = 0; i < n; ++i) chains[i]->impl->stage_scalars(eps[i]);
+      DR_HIP(hipMemcpyAsync(dsc, hsc, n * kSlot, hipMemcpyHostToDevice, stream));
+      DR_HIP(hipGraphLaunch(exec, stream));
+      DR_HIP(hipMemcpyAsync(hsums, dsums, n * drh::NSLOT * sizeof(double), hipMemcpyDeviceToHost, stream));
+      DR_HIP(hipStreamSynchronize(stream));
+      for (size_t i = 0; i < n; ++i) chains[i]->impl->finish_iteration(host_sums(i), u[i], &res[i], true, stream);
+      DR_HIP(hipStreamSynchronize(stream));
+      return;
+    }
+    for (size_t i = 0; i < n; ++i) {
+      auto& k = *chains[i]->impl;
+      if (mode == DR_BATCH_STREAMS) k.enqueue_iteration(eps[i], L[i], k.stream);
+      else k.enqueue_eager(eps[i], L[i]);
+      DR_HIP(hipMemcpyAsync(host_sums(i), dsums + drh::NSLOT * i, drh::NSLOT * sizeof(double),
+                            hipMemcpyDeviceToHost, k.stream));
+    }
+    for (size_t i = 0; i < n; ++i) {
+      auto& k = *chains[i]->impl;
+      DR_HIP(hipStreamSynchronize(k.stream));
+      k.finish_iteration(host_sums(i), u[i], &res[i], mode == DR_BATCH_STREAMS, k.stream);
+    }
+    for (size_t i = 0; i < n; ++i) DR_HIP(hipStreamSynchronize(chains[i]->impl->stream));
+  }
+};
+
+extern "C" {
+
+int dr_batch_create(dr_chain* const* chains, int n, int mode, dr_batch** out) {
+  return guarded([&] {
+    DR_REQUIRE(chains && out && n >= 1, DR_ELOGIC, "dr_batch_create: null argument or empty batch");
+    DR_REQUIRE(mode == DR_BATCH_GRAPH || mode == DR_BATCH_STREAMS || mode == DR_BATCH_EAGER, DR_EDOMAIN,
+               "dr_batch_create: unknown mode");
+    std::unique_ptr<dr_batch> b(new dr_batch());
+    b->mode = mode;
+    b->device = chains[0]->impl->cfg.device;
+    for (int i = 0; i < n; ++i) {
+      DR_REQUIRE(chains[i], DR_ELOGIC, "dr_batch_create: null chain");
+      DR_REQUIRE(chains[i]->impl->cfg.device == b->device, DR_EDOMAIN, "dr_batch_create: chains on different devices");
+      for (int j = 0; j < i; ++j) DR_REQUIRE(chains[j] != chains[i], DR_ELOGIC, "dr_batch_create: chain listed twice");
+    }
+    DR_HIP(hipSetDevice(b->device));
+    DR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    DR_HIP(hipMalloc(&b->dsc, n * dr_batch::kSlot));
+    DR_HIP(hipMalloc((void**)&b->dsums, (size_t)n * drh::NSLOT * sizeof(double)));
+    DR_HIP(hipHostMalloc(&b->hsc, n * dr_batch::kSlot, hipHostMallocDefault));
+    DR_HIP(hipHostMalloc((void**)&b->hsums, (size_t)n * drh::NSLOT * sizeof(double), hipHostMallocDefault));
+    for (int i = 0; i < n; ++i) {
+      b->chains.push_back(chains[i]);
+      chains[i]->impl->set_batch_slots(static_cast<char*>(b->dsc) + i * dr_batch::kSlot,
+                                       static_cast<char*>(b->hsc) + i * dr_batch::kSlot,
+                                       b->dsums + (size_t)drh::NSLOT * i);
+    }
+    *out = b.release();
+  });
+}
+
+int dr_batch_update(dr_batch* b, const double* eps, const int* L, const double* u_accept, dr_update_result* res) {
+  return guarded([&] {
+    DR_REQUIRE(b && eps && L && u_accept && res, DR_ELOGIC, "dr_batch_update: null argument");
+    DR_HIP(hipSetDevice(b->device));
+    b->update(eps, L, u_accept, res);
+  });
+}
+
+int dr_batch_destroy(dr_batch* b) {
+  return guarded([&] { delete b; });
 }
 
 int dr_profile_enable(dr_chain* c, int enable) {

@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "dr_common.hpp"
@@ -378,9 +379,24 @@ __device__ __forceinline__ void fill_grouped(int total, Fetch&& fetch, Put&& put
   }
 }
 
+// Per-iteration scalars (step sizes, noise precisions, noise stream ids) of
+// functors built for a captured HIP graph are read from device memory when a
+// kernel starts: a functor with a bind() member refreshes its by-value
+// fields there; functors without one are left alone.
+template <class F, class = void>
+struct HasBind : std::false_type {};
+template <class F>
+struct HasBind<F, std::void_t<decltype(std::declval<F&>().bind())>> : std::true_type {};
+template <class F>
+__device__ __forceinline__ void bind_scalars(F& f) {
+  if constexpr (HasBind<F>::value) f.bind();
+}
+
 template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
+  bind_scalars(pro);
+  bind_scalars(epi);
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];
@@ -413,6 +429,9 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
 template <typename T, class Pro, class Mid, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid, Epi epi) {
+  bind_scalars(pro);
+  bind_scalars(mid);
+  bind_scalars(epi);
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];
@@ -455,6 +474,9 @@ k_strided2(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Mid mid
 template <typename T, class Pro, class Mid, class Epi, int EPT, int MC>
 __global__ void __launch_bounds__(kThreads)
 k_strided2_pf(AxisFFT<T> ax, int NB, int k_in, int k_out, Pro pro, Mid mid, Epi epi) {
+  bind_scalars(pro);
+  bind_scalars(mid);
+  bind_scalars(epi);
   constexpr int TB = 16, tbs = 4, KS = kThreads / TB, NE = MC / KS;
   constexpr int NP = NE < 32 / (int)sizeof(T) ? NE : 32 / (int)sizeof(T);
   static_assert(NE * KS == MC, "MC must be a multiple of kThreads / 16");
@@ -511,6 +533,8 @@ k_strided2_pf(AxisFFT<T> ax, int NB, int k_in, int k_out, Pro pro, Mid mid, Epi 
 template <typename T, class Pro, class Epi, int EPT = kEptX, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
+  bind_scalars(pro);
+  bind_scalars(epi);
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];
@@ -564,6 +588,8 @@ k_r2c_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
 template <typename T, class Pro, class Epi, int EPT = kEptX, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_c2r_x(AxisFFT<T> ax, int nrows, int tbs, Pro pro, Epi epi) {
+  bind_scalars(pro);
+  bind_scalars(epi);
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   __shared__ double red[kThreads];

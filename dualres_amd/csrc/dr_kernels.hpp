@@ -15,6 +15,17 @@ struct NoRed {
   __device__ __forceinline__ void finish(double*, int) {}
 };
 
+// Per-iteration scalars of one chain in device memory.  The functors and
+// kernels of a captured HMC iteration read them (bind()) in place of their
+// by-value fields, so one graph serves every step size and noise precision.
+// The values are the host's conversions to T, the same ones the by-value
+// fields of an eager launch hold.
+template <typename T> struct StepScalars {
+  T half, full;        // (T)(eps / 2), (T)eps
+  T sig0, sig1;        // sigma_h^-2, sigma_s^-2
+  uint64_t stream[2];  // Philox streams of the momentum draw (real, imaginary chain)
+};
+
 // weight of a half-spectrum entry in a full-torus sum: 1 on the kx = 0 and
 // kx = M0/2 planes, 2 elsewhere (Hermitian partner not stored)
 __device__ __forceinline__ double herm_w(int kx, int H0) { return (kx == 0 || kx == H0 - 1) ? 1.0 : 2.0; }
@@ -58,6 +69,13 @@ template <typename T> struct EpiMomentum {
   const cx<T>* Xi; cx<T>* pim;
   int do_red; double* part; long pstride; int s_prior, s_priori, s_kin, s_kini;
   double a0, a1, a2, a3;
+  const StepScalars<T>* sc;  // half kick
+  __device__ __forceinline__ void bind() {
+    if (sc) {
+      coef = sc->half;
+      sig = sc->sig0;
+    }
+  }
   __device__ __forceinline__ void store(int, int k, int b, cx<T> v) {
     const long i = (long)k * sk + b;
     const T il = invlam[i];
@@ -97,6 +115,14 @@ template <typename T> struct EpiMomentum {
 template <typename T> struct MidLeapfrog {
   const cx<T>* X; cx<T>* Xd; cx<T>* p; const T* invlam; long sk; T coef; T eps; T sig; int H0;
   int do_red; double* part; double a0;
+  const StepScalars<T>* sc; int coef_full;  // coef = full (1) or half (0) step
+  __device__ __forceinline__ void bind() {
+    if (sc) {
+      coef = coef_full ? sc->full : sc->half;
+      eps = sc->full;
+      sig = sc->sig0;
+    }
+  }
   // the state one mode's update reads; each mode is read and written by one
   // thread only, so a thread may fetch its modes' state early
   struct State { cx<T> x, p; T il; };
@@ -129,8 +155,14 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_imag_leapfrog(long Nh, int H0, const cx<T>* __restrict__ muA, cx<T>* __restrict__ muB, cx<T>* __restrict__ p,
                 const T* __restrict__ invlam, T eps, T half_coef, T full_coef, T sig, int L, double* partA,
-                double* partB, double* partK) {
+                double* partB, double* partK, const StepScalars<T>* sc) {
   __shared__ double red[kThreads];
+  if (sc) {
+    eps = sc->full;
+    half_coef = sc->half;
+    full_coef = sc->full;
+    sig = sc->sig0;
+  }
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < Nh; i += (long)gridDim.x * kThreads) {
     const T il = invlam[i];
@@ -167,6 +199,10 @@ template <typename T> struct EpiGrad : NoRed {
 // momentum draw p = sqrt(lambda_m) F(z), 0 on Z (MultiResParameters.h:811-814)
 template <typename T> struct EpiNoiseMomentum : NoRed {
   cx<T>* p; const T* invlam; long sk; T sig;
+  const StepScalars<T>* sc;
+  __device__ __forceinline__ void bind() {
+    if (sc) sig = sc->sig0;
+  }
   __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
     const long i = (long)k * sk + b;
     const T il = invlam[i];
@@ -236,6 +272,10 @@ template <typename T> struct StoreRowR : NoRed {
 // sum of squares of the working-precision values gives E0 (:785-809)
 template <typename T> struct LoadNoise {
   uint64_t seed, stream; int M0h; double* part; double acc;
+  const StepScalars<T>* sc; int which;  // stream = sc->stream[which]
+  __device__ __forceinline__ void bind() {
+    if (sc) stream = sc->stream[which];
+  }
   __device__ __forceinline__ cx<T> load2(int row, int n) {
     double z0, z1;
     normal_pair(seed, stream, (uint64_t)row * (uint64_t)M0h + (uint64_t)n, z0, z1);
@@ -262,6 +302,10 @@ template <typename T> struct LoadNoiseBuf {
 // voxels, 0 elsewhere; accumulates ||Yh - mu[I]||^2 (:502-503).
 template <typename T> struct LoadResidual {
   const T* R; const T* Yg; const uint8_t* mask; const T* WTv; const T* WTvb; T sig; int n0; double* part; double acc;
+  const StepScalars<T>* sc;
+  __device__ __forceinline__ void bind() {
+    if (sc) sig = sc->sig0;
+  }
   // split load: fetch2 reads the inputs of samples 2n, 2n+1 of a row without
   // side effects (any row < nrows, any n: x is clamped into the row), apply2
   // forms the residual and accumulates the SSE for the samples inside the row
@@ -343,8 +387,10 @@ __device__ __forceinline__ T wave_sum(T v) {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_wmu(int n_s, const int* __restrict__ rp, const int* __restrict__ cols, const T* __restrict__ vals,
-      const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, T* __restrict__ v, double* part) {
+      const T* __restrict__ R, const T* __restrict__ Ys, T sig_s, T* __restrict__ v, double* part,
+      const StepScalars<T>* sc) {
   __shared__ double red[kThreads];
+  if (sc) sig_s = sc->sig1;
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   double e2 = 0.0;
@@ -565,8 +611,9 @@ __device__ __forceinline__ double wmu_far_block(const WmuFar<T>& a, int blk, con
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_wmu_sb(WmuInt<T> ai, WmuBnd<T> ab, WmuFar<T> af, int gi, int gb, const T* __restrict__ R,
-         const T* __restrict__ Ys, T sig_s, double* part) {
+         const T* __restrict__ Ys, T sig_s, double* part, const StepScalars<T>* sc) {
   __shared__ double red[kThreads];
+  if (sc) sig_s = sc->sig1;
   const int blk = blockIdx.x;
   const double e2 = blk < gi        ? wmu_int_block(ai, blk, R, Ys, sig_s)
                     : blk < gi + gb ? wmu_bnd_block(ab, blk - gi, R, Ys, sig_s)

@@ -200,6 +200,12 @@ class Chain:
                                                    L.dptr(zr), L.dptr(zi), C.byref(res)))
         return res.as_dict()
 
+    def update_captured(self, eps, L_steps, u_accept):
+        """update() through the chain's captured HIP graph for this leapfrog length."""
+        res = L.dr_update_result()
+        L.check(self._lib.dr_update_captured(self._h, float(eps), int(L_steps), float(u_accept), C.byref(res)))
+        return res.as_dict()
+
     def leapfrog(self, eps, L_steps):
         res = L.dr_update_result()
         L.check(self._lib.dr_leapfrog(self._h, float(eps), int(L_steps), C.byref(res)))

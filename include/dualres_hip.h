@@ -169,6 +169,35 @@ int dr_update(dr_chain* c, double eps, int L, double u_accept, dr_update_result*
 int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept,
                          const double* z_re, const double* z_im, dr_update_result* res);
 
+/* dr_update through a captured HIP graph: the iteration's device work
+ * (momentum draw, the L leapfrog steps, the energy sums) is captured once per
+ * leapfrog length and replayed, reading the step size, noise precisions and
+ * noise stream of the iteration from device memory.  Same kernels, scalars
+ * and results as dr_update, bit for bit.  No counterpart in the reference
+ * (its L-step loop is MultiResParameters.h:691-700). */
+int dr_update_captured(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res);
+
+/* --- batches: many chains of uneven size advanced together --------------- */
+enum dr_batch_mode {
+  DR_BATCH_GRAPH = 0,   /* one graph per iteration: every chain's captured
+                           iteration a parallel branch, one launch          */
+  DR_BATCH_STREAMS = 1, /* each chain's captured iteration on its own stream */
+  DR_BATCH_EAGER = 2    /* plain kernel launches on each chain's stream      */
+};
+typedef struct dr_batch dr_batch;
+/* The chains (created, data set, on one device) are borrowed until
+ * dr_batch_destroy and must not be updated on their own meanwhile; their
+ * other calls (sigma, summaries, outputs) stay available between batch
+ * updates.  No counterpart in the reference, which runs one chain per
+ * process (gaussian_process_model.h:87-177). */
+int dr_batch_create(dr_chain* const* chains, int n, int mode, dr_batch** out);
+/* One HMC iteration of every chain i with step size eps[i], leapfrog length
+ * L[i] and accept uniform u_accept[i]; res[i] as dr_update's.  A chain's
+ * results are the same bit for bit in any batch and mode and through
+ * dr_update_captured. */
+int dr_batch_update(dr_batch* b, const double* eps, const int* L, const double* u_accept, dr_update_result* res);
+int dr_batch_destroy(dr_batch* b);
+
 /* Energies of the current state: log_prior (:428-476), log_likelihood
  * (:488-514), log_posterior (:528-544). */
 int dr_log_prior(dr_chain* c, double* out);
