@@ -110,6 +110,8 @@ SIGNATURES = {
     "dr_csr_free": [_P],
     "dr_mce_offsets": [_I64, _I32],
     "dr_mce_summary": [_D, C.POINTER(C.c_int), _F, C.c_int, _F, _F, _I32],
+    "dr_mce_summary_noalloc": [C.c_void_p, C.POINTER(C.c_int), _F, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p],
     "dr_mce_binned": [_D, C.POINTER(C.c_int), _F, C.c_int, C.c_int, C.c_double, _D, _D, _I64],
 }
 

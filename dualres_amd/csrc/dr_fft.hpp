@@ -649,14 +649,27 @@ std::vector<cx<T>> twiddles(int M) {
   return t;
 }
 
+// Stage radices in factorize()'s order, into the descriptor's fixed array
+// (no heap use, so the allocation-free entry points can build plans).
 template <typename T>
 AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws) {
   AxisFFT<T> a{};
   a.M = fft_len;
-  std::vector<int> f = factorize(fft_len);
-  if ((int)f.size() > kMaxStages) throw Error(DR_EDOMAIN, "FFT length needs too many stages");
-  a.nst = (int)f.size();
-  for (int i = 0; i < a.nst; ++i) a.radix[i] = f[i];
+  if (fft_len < 1) throw Error(DR_EDOMAIN, "FFT length must be positive");
+  int m = fft_len;
+  auto take = [&](int r) {
+    if (a.nst >= kMaxStages) throw Error(DR_EDOMAIN, "FFT length needs too many stages");
+    a.radix[a.nst++] = r;
+    m /= r;
+  };
+  while (m % 16 == 0) take(16);
+  while (m % 8 == 0 && m > 8) take(8);
+  if (m % 8 == 0) take(8);
+  while (m % 4 == 0) take(4);
+  while (m % 2 == 0) take(2);
+  for (int r : {3, 5, 7})
+    while (m % r == 0) take(r);
+  if (m != 1) throw Error(DR_EDOMAIN, "FFT length " + std::to_string(fft_len) + " has a prime factor > 7");
   a.tw = tw;
   a.tws = tws;
   return a;

@@ -13,6 +13,8 @@
 // reference, and pair counts are the rounded m*m correlation (exact).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <unordered_set>
 #include <vector>
 
@@ -94,30 +96,28 @@ int smooth_even(int n) {
   }
 }
 
-// float64 3D transforms on one torus (full, unpruned)
+// float64 3D transforms on one torus (full, unpruned).  Buffers (twiddle
+// tables, pass scratch T1/T2) are borrowed: plan() only records them.
 struct Fft3d {
   int M[3] = {0, 0, 0}, H0 = 0;
   long N = 0, Nh = 0;
   hipStream_t s = nullptr;
-  MBuf<cx<double>> tw[3];
   AxisFFT<double> ax[3]{};
-  MBuf<cx<double>> T1, T2;
+  cx<double>* T1 = nullptr;
+  cx<double>* T2 = nullptr;
 
-  void init(const int m[3], hipStream_t st) {
+  static void sizes(const int m[3], long* N, long* Nh) {
+    *N = (long)m[0] * m[1] * m[2];
+    *Nh = (long)(m[0] / 2 + 1) * m[1] * m[2];
+  }
+  void plan(const int m[3], hipStream_t st, cx<double>* const tw[3], cx<double>* t1, cx<double>* t2) {
     s = st;
     for (int a = 0; a < 3; ++a) M[a] = m[a];
     H0 = M[0] / 2 + 1;
-    N = (long)M[0] * M[1] * M[2];
-    Nh = (long)H0 * M[1] * M[2];
-    for (int a = 0; a < 3; ++a) {
-      auto t = twiddles<double>(M[a]);
-      tw[a].alloc(M[a]);
-      DR_HIP(hipMemcpyAsync(tw[a].p, t.data(), M[a] * sizeof(cx<double>), hipMemcpyHostToDevice, s));
-      DR_HIP(hipStreamSynchronize(s));
-      ax[a] = make_axis<double>(a == 0 ? M[0] / 2 : M[a], tw[a].p, a == 0 ? 2 : 1);
-    }
-    T1.alloc(Nh);
-    T2.alloc(Nh);
+    sizes(M, &N, &Nh);
+    for (int a = 0; a < 3; ++a) ax[a] = make_axis<double>(a == 0 ? M[0] / 2 : M[a], tw[a], a == 0 ? 2 : 1);
+    T1 = t1;
+    T2 = t2;
   }
   template <bool INV, class Pro, class Epi>
   void strided(int a, int n_outer, int NB, Pro pro, Epi epi) {
@@ -139,18 +139,18 @@ struct Fft3d {
     const int rows = M[1] * M[2], tbs = xtbs(rows), TB = 1 << tbs, Mh = ax[0].M;
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<double>);
     hipLaunchKernelGGL((k_r2c_x<double, Pro, StoreRowC<double>>), dim3(cdiv(rows, TB)), dim3(kThreads), lds, s, ax[0],
-                       rows, tbs, pro, StoreRowC<double>{{}, T2.p, H0});
+                       rows, tbs, pro, StoreRowC<double>{{}, T2, H0});
     DR_HIP(hipGetLastError());
-    strided<false>(1, M[2], H0, LoadS<double>{{}, T2.p, (long)M[1] * H0, H0}, StoreS<double>{{}, T1.p, (long)M[1] * H0, H0});
-    strided<false>(2, 1, M[1] * H0, LoadS<double>{{}, T1.p, 0, (long)M[1] * H0}, StoreS<double>{{}, X, 0, (long)M[1] * H0});
+    strided<false>(1, M[2], H0, LoadS<double>{{}, T2, (long)M[1] * H0, H0}, StoreS<double>{{}, T1, (long)M[1] * H0, H0});
+    strided<false>(2, 1, M[1] * H0, LoadS<double>{{}, T1, 0, (long)M[1] * H0}, StoreS<double>{{}, X, 0, (long)M[1] * H0});
   }
   void c2r(const cx<double>* X, double* dst) {
-    strided<true>(2, 1, M[1] * H0, LoadS<double>{{}, X, 0, (long)M[1] * H0}, StoreS<double>{{}, T1.p, 0, (long)M[1] * H0});
-    strided<true>(1, M[2], H0, LoadS<double>{{}, T1.p, (long)M[1] * H0, H0}, StoreS<double>{{}, T2.p, (long)M[1] * H0, H0});
+    strided<true>(2, 1, M[1] * H0, LoadS<double>{{}, X, 0, (long)M[1] * H0}, StoreS<double>{{}, T1, 0, (long)M[1] * H0});
+    strided<true>(1, M[2], H0, LoadS<double>{{}, T1, (long)M[1] * H0, H0}, StoreS<double>{{}, T2, (long)M[1] * H0, H0});
     const int rows = M[1] * M[2], tbs = xtbs(rows), TB = 1 << tbs, Mh = ax[0].M;
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<double>);
     hipLaunchKernelGGL((k_c2r_x<double, LoadRowC<double>, StoreRowR<double>>), dim3(cdiv(rows, TB)), dim3(kThreads),
-                       lds, s, ax[0], rows, tbs, LoadRowC<double>{{}, T2.p, H0},
+                       lds, s, ax[0], rows, tbs, LoadRowC<double>{{}, T2, H0},
                        StoreRowR<double>{{}, dst, M[0], 1.0 / (double)N});
     DR_HIP(hipGetLastError());
   }
@@ -200,8 +200,19 @@ static std::vector<double> corr_sums(const double* img, const int dims[3], const
   hipStream_t s;
   DR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   struct SG { hipStream_t s; ~SG() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{s};
+  long N, Nh;
+  Fft3d::sizes(M, &N, &Nh);
+  MBuf<cx<double>> tw[3], T1(Nh), T2(Nh);
+  cx<double>* twp[3];
+  for (int a = 0; a < 3; ++a) {
+    const auto t = twiddles<double>(M[a]);
+    tw[a].alloc(M[a]);
+    DR_HIP(hipMemcpyAsync(tw[a].p, t.data(), M[a] * sizeof(cx<double>), hipMemcpyHostToDevice, s));
+    DR_HIP(hipStreamSynchronize(s));
+    twp[a] = tw[a].p;
+  }
   Fft3d f;
-  f.init(M, s);
+  f.plan(M, s, twp, T1.p, T2.p);
   const long nvox = (long)dims[0] * dims[1] * dims[2];
   MBuf<double> d_img(nvox);
   DR_HIP(hipMemcpyAsync(d_img.p, img, nvox * sizeof(double), hipMemcpyHostToDevice, s));
@@ -245,6 +256,121 @@ static float offset_distance(const float q[16], const int* p) {
   return sqrt_rn(ss);
 }
 
+// ---------------------------------------------------------------- allocation-free summary
+// dr_mce_summary_noalloc keeps every intermediate in one fixed device array
+// per GPU, `scratch`, and the offset table in `mce_offset_table`.  Host code
+// never passes these names to a kernel: on each device the first call looks
+// up their device addresses with hipGetSymbolAddress (that device current)
+// and every later use goes through the recorded pointers.
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+constexpr int kNumMceOffsets = 25348;
+constexpr int kMaxNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static __device__ __align__(256) int mce_offset_table[3 * kNumMceOffsets];
+
+struct NoAllocDevice {
+  bool ready = false;
+  unsigned char* scr = nullptr;  // device address of `scratch` on this device
+  int* off = nullptr;            // device address of `mce_offset_table`
+};
+static NoAllocDevice g_noalloc[kMaxNoAllocDevices];
+static std::mutex g_noalloc_mu;
+
+// one-time set-up on `device` (current): symbol addresses and the offset table
+static const NoAllocDevice& noalloc_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocDevice& d = g_noalloc[device];
+  if (d.ready) return d;
+  void* p = nullptr;
+  DR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  d.scr = static_cast<unsigned char*>(p);
+  p = nullptr;
+  DR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(mce_offset_table)));
+  d.off = static_cast<int*>(p);
+  const std::vector<int> P = mce_offsets();
+  DR_REQUIRE((long)P.size() == 3L * kNumMceOffsets, DR_ELOGIC, "minimum contrast: offset table size changed");
+  DR_HIP(hipMemcpy(d.off, P.data(), P.size() * sizeof(int), hipMemcpyHostToDevice));
+  d.ready = true;
+  return d;
+}
+
+namespace {
+
+// exp(-2 pi i e / M): cos and sin of pi * (2e/M), the argument reduced exactly
+__global__ void k_twiddles(int M, cx<double>* __restrict__ t) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double sn, cs;
+  sincospi(2.0 * (double)e / (double)M, &sn, &cs);
+  t[e] = mk<double>(cs, -sn);
+}
+
+// one of the four products conj(A) B of k_corr_products
+__global__ void k_corr_product_one(long Nh, const cx<double>* __restrict__ X, const cx<double>* __restrict__ Mk,
+                                   int k, cx<double>* __restrict__ P) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Nh) return;
+  const cx<double> x = X[i], m = Mk[i];
+  const cx<double> a = (k == 0 || k == 2) ? x : m;
+  const cx<double> b = (k == 0 || k == 1) ? x : m;
+  P[i] = cmulc(a, b);
+}
+
+// column k of k_gather_offsets
+__global__ void k_gather_one(int n, const int* __restrict__ off, int M0, int M1, int M2,
+                             const double* __restrict__ R, int k, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = ((off[3 * i] % M0) + M0) % M0;
+  const int y = ((off[3 * i + 1] % M1) + M1) % M1;
+  const int z = ((off[3 * i + 2] % M2) + M2) % M2;
+  out[4 * i + k] = R[x + (long)M0 * (y + (long)M1 * z)];
+}
+
+struct QForm {
+  float q[16];
+};
+
+// dr_mce_summary's host loop on the device, with the same rounding: pair
+// counts rounded, (Sxy - Sx Sy / n) / (n - 1) in float64 then float, and the
+// float distance ||[p, 0] Q^T|| with rows accumulated left to right
+__global__ void k_mce_finalize(int n, const double* __restrict__ sums, const int* __restrict__ off, QForm qf,
+                               float* __restrict__ cov, float* __restrict__ dist, int32_t* __restrict__ npairs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sxy = sums[4 * i], sx = sums[4 * i + 1], sy = sums[4 * i + 2];
+  const long np = (long)rint(sums[4 * i + 3]);
+  npairs[i] = (int32_t)np;
+  cov[i] = np > 1 ? (float)__ddiv_rn(__dsub_rn(sxy, __ddiv_rn(__dmul_rn(sx, sy), (double)np)), (double)(np - 1))
+                  : 0.0f;
+  float ss = 0.0f;
+  for (int r = 0; r < 4; ++r) {
+    float v = 0.0f;
+    for (int c = 0; c < 3; ++c) v = __fadd_rn(v, __fmul_rn(qf.q[4 * r + c], (float)off[3 * i + c]));
+    ss = __fadd_rn(ss, __fmul_rn(v, v));
+  }
+  dist[i] = sqrt_rn(ss);
+}
+
+}  // namespace
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Scratch bytes the allocation-free summary needs for an image of `dims`
+// (twiddles, T1, T2, the two forward transforms, one product, one
+// correlation grid, the per-offset sums); fills the padded torus M.
+static size_t noalloc_bytes(const int dims[3], int M[3]) {
+  for (int a = 0; a < 3; ++a) M[a] = smooth_even(dims[a] + 25 + 1);  // largest offset component: 25
+  long N, Nh;
+  Fft3d::sizes(M, &N, &Nh);
+  size_t b = 0;
+  for (int a = 0; a < 3; ++a) b += align256((size_t)M[a] * sizeof(cx<double>));
+  b += 5 * align256((size_t)Nh * sizeof(cx<double>));
+  b += align256((size_t)N * sizeof(double));
+  b += align256((size_t)4 * kNumMceOffsets * sizeof(double));
+  return b;
+}
+
 }  // namespace drh
 
 extern "C" {
@@ -272,6 +398,67 @@ int dr_mce_summary(const double* img, const int dims[3], const float qform[16], 
       covariance[i] = np > 1 ? (float)((sxy - sx * sy / (double)np) / (double)(np - 1)) : 0.0f;
       distance[i] = drh::offset_distance(qform, &P[3 * i]);
     }
+  });
+}
+
+int dr_mce_summary_noalloc(const double* d_img, const int dims[3], const float qform[16], int device, void* stream,
+                           float* d_covariance, float* d_distance, int32_t* d_npairs) {
+  return drh::guarded([&] {
+    using namespace drh;
+    DR_REQUIRE(d_img && dims && qform && d_covariance && d_distance && d_npairs, DR_ELOGIC,
+               "dr_mce_summary_noalloc: null argument");
+    for (int a = 0; a < 3; ++a) DR_REQUIRE(dims[a] >= 1, DR_EDOMAIN, "minimum contrast: empty image");
+    DR_REQUIRE(device >= 0 && device < kMaxNoAllocDevices, DR_EDOMAIN, "dr_mce_summary_noalloc: bad device");
+    int M[3];
+    const size_t need = noalloc_bytes(dims, M);
+    if (need > kNoAllocScratchBytes)
+      throw Error(DR_EDOMAIN, "dr_mce_summary_noalloc: image " + std::to_string(dims[0]) + "x" +
+                                  std::to_string(dims[1]) + "x" + std::to_string(dims[2]) + " needs " +
+                                  std::to_string(need) + " bytes of scratch; the limit is " +
+                                  std::to_string(kNoAllocScratchBytes));
+    DR_HIP(hipSetDevice(device));
+    const NoAllocDevice& d = noalloc_setup(device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    long N, Nh;
+    Fft3d::sizes(M, &N, &Nh);
+    unsigned char* cur = d.scr;
+    auto carve = [&](size_t bytes) {
+      unsigned char* p = cur;
+      cur += align256(bytes);
+      return p;
+    };
+    cx<double>* tw[3];
+    for (int a = 0; a < 3; ++a) tw[a] = reinterpret_cast<cx<double>*>(carve((size_t)M[a] * sizeof(cx<double>)));
+    auto cbuf = [&]() { return reinterpret_cast<cx<double>*>(carve((size_t)Nh * sizeof(cx<double>))); };
+    cx<double>* T1 = cbuf();
+    cx<double>* T2 = cbuf();
+    cx<double>* X = cbuf();
+    cx<double>* Mk = cbuf();
+    cx<double>* P = cbuf();
+    double* R = reinterpret_cast<double*>(carve((size_t)N * sizeof(double)));
+    double* sums = reinterpret_cast<double*>(carve((size_t)4 * kNumMceOffsets * sizeof(double)));
+    DR_REQUIRE((size_t)(cur - d.scr) == need, DR_ELOGIC, "dr_mce_summary_noalloc: scratch layout mismatch");
+    for (int a = 0; a < 3; ++a) {
+      hipLaunchKernelGGL(k_twiddles, dim3(cdiv(M[a], 256)), dim3(256), 0, s, M[a], tw[a]);
+      DR_HIP(hipGetLastError());
+    }
+    Fft3d f;
+    f.plan(M, s, tw, T1, T2);
+    f.r2c(LoadMasked{{}, d_img, dims[0], dims[1], dims[2], M[1], 0}, X);
+    f.r2c(LoadMasked{{}, d_img, dims[0], dims[1], dims[2], M[1], 1}, Mk);
+    for (int k = 0; k < 4; ++k) {
+      hipLaunchKernelGGL(k_corr_product_one, dim3(cdiv(Nh, 256)), dim3(256), 0, s, Nh, X, Mk, k, P);
+      DR_HIP(hipGetLastError());
+      f.c2r(P, R);
+      hipLaunchKernelGGL(k_gather_one, dim3(cdiv(kNumMceOffsets, 256)), dim3(256), 0, s, kNumMceOffsets, d.off, M[0],
+                         M[1], M[2], R, k, sums);
+      DR_HIP(hipGetLastError());
+    }
+    QForm qf;
+    for (int i = 0; i < 16; ++i) qf.q[i] = qform[i];
+    hipLaunchKernelGGL(k_mce_finalize, dim3(cdiv(kNumMceOffsets, 256)), dim3(256), 0, s, kNumMceOffsets, sums, d.off,
+                       qf, d_covariance, d_distance, d_npairs);
+    DR_HIP(hipGetLastError());
   });
 }
 

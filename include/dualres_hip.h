@@ -249,6 +249,22 @@ int dr_mce_offsets(int64_t* n, int32_t* offsets_xyz /* may be NULL; else n*3 */)
  * covariance (float), distance (float), npairs (int). */
 int dr_mce_summary(const double* img, const int dims[3], const float qform[16], int device,
                    float* covariance, float* distance, int32_t* npairs);
+/* Allocation-free dr_mce_summary.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  d_img is a float64 volume in DEVICE memory
+ * (x fastest, 0/NaN missing); d_covariance, d_distance and d_npairs are
+ * caller-allocated DEVICE arrays with one entry per MCE offset
+ * (dr_mce_offsets, 25,348), written with dr_mce_summary's values.  All
+ * intermediates live in one fixed scratch array of 48 MiB per device, held
+ * until the process ends: images whose zero-padded torus needs more (about
+ * 70^3 voxels and up) are rejected with DR_EDOMAIN, and no device memory is
+ * ever allocated or freed.  Work is enqueued on `stream` (a hipStream_t;
+ * NULL = the null stream) and the call returns without waiting.  The first
+ * call on a device also looks up the scratch address and uploads the offset
+ * table (synchronously), so it must not be made under graph capture.  Calls
+ * that overlap on one device race on the shared scratch. */
+int dr_mce_summary_noalloc(const double* d_img, const int dims[3], const float qform[16], int device, void* stream,
+                           float* d_covariance, float* d_distance, int32_t* d_npairs);
 /* Binned empirical covariance: nbins distance bins on [0, maxdist] over
  * all offsets with |offset| <= maxdist (FP64 sums per bin). */
 int dr_mce_binned(const double* img, const int dims[3], const float qform[16], int device,
