@@ -231,39 +231,72 @@ def serving_leg(rates=(4.0, 8.0, 16.0), n_req=32, workers=4, seed=20261202, n_im
             "40+40 iterations, L=25", "rates": out}
 
 
-def batch_leg(n_chains=32, workers=8, seed=7):
-    """Many uneven dual-resolution chains at once through the fit server:
-    highres dims U{32..112} per axis at 1.5 mm (stdres 3 mm), theta =
-    (1, U(0.08, 0.2), U(0.8, 1.8)), 6.9 mm neighbourhood, smallest even
-    2,3,5,7-smooth embedding >= 2 max ijk per axis, 30 + 30 iterations of
-    L = 25 in FP32, all submitted together."""
-    from dualres_amd import problem, serve
+def batch_problems(n_chains=64, seed=7):
+    """The batched-chains workload: n_chains dual-resolution images, highres
+    dims drawn per axis from U{32..112} at 1.5 mm (stdres 3 mm block means),
+    their own ellipsoid masks, theta = (1, U(0.08, 0.2), U(0.8, 1.8)) in the
+    (tau^2, psi, nu) order, kriging W built on the device with a 6.9 mm
+    neighbourhood, embedding per axis the smallest even 2,3,5,7-smooth length
+    >= 2 max ijk."""
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
     rng = np.random.default_rng(seed)
-    reqs = []
+    out = []
+    t = time.perf_counter()
     for i in range(n_chains):
-        dims = tuple(2 * int(rng.integers(16, 57)) for _ in range(3))
+        dims = tuple(int(rng.integers(32, 113)) for _ in range(3))
         th = (1.0, float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 1.8)))
         q = problem.dual_problem(dims, 1.5, th, seed_gp=500 + i)
-        mx = q["ijk"].max(0)
-        emb = tuple(problem.smooth_even(2 * int(m)) for m in mx)
-        reqs.append(serve.make_request(q["hr"], q["q_hr"], th, std=q["std"], q_std=q["q_std"], neighborhood=6.9,
-                                       burnin=30, nsave=30, thin=1, L=L_STEPS, seed=900 + i, precision="fp32",
-                                       embed=emb))
-    with serve.Server(200 * 10 ** 9, workers=workers) as srv:
-        t0 = time.perf_counter()
-        ids = [srv.submit(r) for r in reqs]
-        res = [srv.wait(i, timeout=900) for i in ids]
-        wall = time.perf_counter() - t0
-    bad = [x["error"] for x in res if x["status"] != 0]
-    if bad:
-        raise RuntimeError(f"batched fit failed: {bad[0]}")
-    grads = sum(x["grad_evals"] for x in res)
-    out = {"chains": n_chains, "workers": workers, "wall_s": round(wall, 2),
-           "post_burnin_grad_evals_per_s": round(grads / wall, 1), "fits_per_s": round(n_chains / wall, 3),
-           "workload": "dual-res, highres U{32..112}^3 @1.5mm, 6.9mm neighbourhood, smooth embeddings, fp32, "
-                       "30+30 iterations, L=25"}
-    log(f"[bench] batch of {n_chains} chains: {out['post_burnin_grad_evals_per_s']} post-burn-in grad evals/s, "
-        f"{wall:.1f} s")
+        q["W"] = kriging_build(q["hr"], q["q_hr"], q["std"], q["q_std"], th, 6.9)
+        q["theta"] = th
+        q["embed"] = tuple(problem.smooth_even(2 * int(m)) for m in q["ijk"].max(0))
+        out.append(q)
+    log(f"[bench] batch workload: {n_chains} images and kriging builds in {time.perf_counter() - t:.1f}s")
+    return out
+
+
+def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, seed=900):
+    """Post-burn-in grad evals/s of all chains fitted in lockstep through one
+    Batch (fit_batch: per chain dual averaging, Gibbs draws and saves; FP32
+    gradients, FP64 energies), L = 25, in each batch mode; a chain's posterior
+    mean must be the same bit for bit in every mode."""
+    from dualres_amd import Chain
+    from dualres_amd.batch import fit_batch
+    from dualres_amd.sampler import HMCParameters
+    out = {"chains": len(probs), "workload": "64 dual-res images, highres U{32..112}^3 @1.5mm (stdres 3mm), "
+           "theta (1, U(0.08,0.2), U(0.8,1.8)), 6.9mm neighbourhood, smooth embeddings, fp32 gradients, fp64 "
+           f"energies, L=25, {burnin} burn-in + {timed} timed iterations", "modes": {}}
+    means = {}
+    for mode in modes:
+        chains = []
+        for q in probs:
+            c = Chain(q["ijk"], q["theta"], q["q_hr"], n_datasets=2, precision="fp32", state_mode="compat",
+                      embed=q["embed"], seed=11)
+            c.set_data(q["Yh"], q["Ys"], q["W"])
+            c.add_data_to_mu()
+            c.set_output_indices(q["ijk"])
+            chains.append(c)
+        hmcs = [HMCParameters(burnin=burnin, n_save=timed, thin=1, integrator_steps=L_STEPS,
+                              starting_eps=1.0 / L_STEPS) for _ in probs]
+
+        def tick(it, timed_it):
+            if it % 10 == 0:
+                log(f"[bench] batch {mode}: iteration {it}")
+        summ, stats, info = fit_batch(chains, hmcs, [1.0] * len(probs), [q["Ys"].size for q in probs],
+                                      [seed + i for i in range(len(probs))], mode=mode, timer=tick)
+        rate = info["post_burnin_grad_evals"] / info["wall_s"]
+        out["modes"][mode] = {"post_burnin_grad_evals_per_s": round(rate, 1), "wall_s": round(info["wall_s"], 3),
+                              "ms_per_iteration": round(1e3 * info["wall_s"] / timed, 3)}
+        means[mode] = [s.mode_mu() for s in summ]
+        log(f"[bench] batch {mode}: {rate:.1f} post-burn-in grad evals/s, {1e3 * info['wall_s'] / timed:.2f} ms "
+            f"per iteration of {len(probs)} chains")
+        for c in chains:
+            c.close()
+    ref = means[modes[0]]
+    out["bitwise_equal_across_modes"] = all(all(np.array_equal(a, b) for a, b in zip(ref, means[m])) for m in modes)
+    best = max(out["modes"], key=lambda m: out["modes"][m]["post_burnin_grad_evals_per_s"])
+    out["best_mode"] = best
+    out["post_burnin_grad_evals_per_s"] = out["modes"][best]["post_burnin_grad_evals_per_s"]
     return out
 
 
@@ -368,6 +401,7 @@ def main():
                     help="also run a plain 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the --precision complex-state chain (for profiler runs)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched-chains leg and print its JSON")
     ap.add_argument("--cpu-study", action="store_true",
                     help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
                          "medians of 3 iterations) and print its JSON")
@@ -385,6 +419,9 @@ def main():
     p = build_problem()
     if a.cpu_study:
         print(json.dumps(cpu_study(p, eps=0.54)), flush=True)
+        return
+    if a.batch_only:
+        print(json.dumps(batch_leg(batch_problems())), flush=True)
         return
     results = {}
     main_chain = None
@@ -421,7 +458,7 @@ def main():
         calibration_copy()
     serving = None if a.main_only or a.no_serve else serving_leg(
         rates=tuple(float(x) for x in a.serve_rates.split(",")), n_req=a.serve_requests)
-    batch = None if a.main_only or a.no_serve else batch_leg()
+    batch = None if a.main_only or a.no_serve else batch_leg(batch_problems())
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(p, eps=results[a.precision]["eps"])

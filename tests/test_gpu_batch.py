@@ -32,14 +32,16 @@ def _same(a, b):
 @pytest.mark.parametrize("dual", [False, True])
 def test_captured_update_is_bitwise_the_eager_update(gpu_available, precision, dual):
     """Eight iterations with varying step sizes, leapfrog lengths, noise
-    precisions and accept uniforms (some rejected): energies, accept
+    precisions and accept uniforms (half accepted whatever the energy, the
+    others rejected unless it fell): energies, accept
     decisions and the kept state equal dr_update's bit for bit."""
     p = small_dual(dims=(16, 14, 12)) if dual else small_single(dims=(14, 12, 10))
     a, b = _chain(p, precision, 9), _chain(p, precision, 9)
     rng = np.random.default_rng(4)
     n_acc = 0
     for it in range(8):
-        eps, L, u = rng.uniform(0.05, 0.4), int(rng.integers(1, 7)), rng.uniform() if it % 3 else 0.9999
+        # u = 0 accepts any finite proposal; u = 0.9999 rejects unless the energy rose
+        eps, L, u = rng.uniform(0.02, 0.15), int(rng.integers(1, 7)), 0.0 if it % 2 == 0 else 0.9999
         r1 = a.update(eps, L, u)
         r2 = b.update_captured(eps, L, u)
         _same(r1, r2)
@@ -115,3 +117,31 @@ def test_batch_of_one_and_reuse_after_batch(gpu_available):
     assert np.array_equal(a.mu(), b.mu())
     a.close()
     b.close()
+
+
+def test_fit_batch_equals_fit_alone(gpu_available):
+    """fit_batch (lockstep fits through one graph per iteration) against fit()
+    on each chain alone: same dual-averaged step sizes, posterior means and
+    variances, bit for bit, over burn-in (eps search with L = 1, then dual
+    averaging) and saved iterations."""
+    from dualres_amd.batch import fit_batch
+    from dualres_amd.sampler import HMCParameters, fit
+    probs = _uneven_problems()[:3]
+    seeds = [71, 72, 73]
+    n_s = [p["Ys"].size if "Ys" in p else 0 for p in probs]
+
+    def hmc():
+        return HMCParameters(burnin=8, n_save=5, thin=1, integrator_steps=5, starting_eps=0.05)
+
+    alone = [_chain(p, "fp32", 40 + i) for i, p in enumerate(probs)]
+    inb = [_chain(p, "fp32", 40 + i) for i, p in enumerate(probs)]
+    ref = [fit(c, hmc(), THETA[0], n_s=n_s[i], seed=seeds[i]) for i, c in enumerate(alone)]
+    hb = [hmc() for _ in probs]
+    summ, stats, info = fit_batch(inb, hb, [THETA[0]] * 3, n_s, seeds, mode="graph")
+    assert info["post_burnin_grad_evals"] > 0
+    for (sa, st), sb, stb in zip(ref, summ, stats):
+        assert st["eps"] == stb["eps"]
+        assert np.array_equal(sa.mode_mu(), sb.mode_mu())
+        assert np.array_equal(sa.var_mu(), sb.var_mu())
+    for c in alone + inb:
+        c.close()
