@@ -178,30 +178,72 @@ def calibration_copy(nbytes=256 * 2 ** 20, reps=3):
     log(f"[bench] calibration copy: {reps} x {nbytes} bytes device to device")
 
 
-def serving_leg(rates=(4.0, 8.0, 16.0), n_req=32, workers=4, seed=20261202, n_images=32):
-    """Mixed fits through the fit server: Poisson arrivals at each rate, half
-    single- and half dual-resolution, highres 48^3 or 64x80x64 at 1.5 mm,
-    theta = (1, U(0.08, 0.2), U(0.8, 1.8)), 40 burn-in + 40 saved iterations of
-    L = 25 in FP32.  Requests cycle over `n_images` synthetic images, each
-    request with its own HMC seed.  Returns p50/p99 submit-to-result latency
-    per rate."""
+def engine_gp_draw(dims, vox, theta, seed):
+    """A GP draw on the whole dims box from the engine itself: a chain's
+    initial state is the circulant-embedded draw scaled by 1/10
+    (MultiResParameters.h:743-759); float32, embedding 2 dims (smooth)."""
+    from dualres_amd import Chain, problem
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    emb = tuple(problem.smooth_even(2 * d) for d in dims)
+    c = Chain(ijk, theta, problem.qform_axis(vox, dims), n_datasets=1, precision="fp32", state_mode="real",
+              embed=emb, seed=seed)
+    c.set_data(np.zeros(ijk.shape[0]))
+    draw = 10.0 * c.mu()
+    c.close()
+    return draw.reshape(nz, ny, nx)
+
+
+SERVE_SIZES = ((48, 48, 48), (64, 80, 64), (96, 112, 96), (128, 160, 128))
+
+
+def serving_requests(n_req, nsave, burnin, seed=20261202, n_images=16):
+    """The serving workload: highres grids cycling over 48^3, 64x80x64,
+    96x112x96 and 128x160x128 at 1.5 mm, half single- and half
+    dual-resolution (stdres 3 mm block means + N(0, 0.5^2)), ellipsoid masks,
+    theta = (1, U(0.08, 0.2), U(0.8, 1.8)), N(0, 1) noise; FP32 fits of L = 25,
+    6.9 mm neighbourhood, smooth embeddings.  Requests cycle over n_images
+    images, each with its own HMC seed."""
     from dualres_amd import problem, serve
     rng = np.random.default_rng(seed)
     images = []
+    t = time.perf_counter()
     for i in range(min(n_req, n_images)):
-        dims = (48, 48, 48) if i % 4 < 2 else (64, 80, 64)
+        dims = SERVE_SIZES[(i // 2) % len(SERVE_SIZES)]
         th = (1.0, float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 1.8)))
-        if i % 2 == 0:
-            q = problem.single_problem(dims, 1.5, th, seed_gp=100 + i)
-            images.append((q["hr"], q["q_hr"], th, {}))
-        else:
-            q = problem.dual_problem(dims, 1.5, th, seed_gp=100 + i)
-            images.append((q["hr"], q["q_hr"], th, {"std": q["std"], "q_std": q["q_std"]}))
+        f = engine_gp_draw(dims, 1.5, th, seed=100 + i)
+        m = problem.ellipsoid_mask(dims, 0.45)
+        hr = np.where(m, problem.nudge_zeros(f + rng.standard_normal(f.shape), m), 0.0)
+        q_hr = problem.qform_axis(1.5, dims)
+        ijk = problem.nonzero_ijk(hr)
+        emb = tuple(problem.smooth_even(2 * int(v)) for v in (ijk.max(0) - ijk.min(0)))
+        extra = {}
+        if i % 2:
+            sdims = tuple(d // 2 for d in dims)
+            sm = problem.ellipsoid_mask(sdims, 0.45)
+            s = problem.block_means(f, 2) + 0.5 * rng.standard_normal(sm.shape)
+            extra = {"std": np.where(sm, problem.nudge_zeros(s, sm), 0.0),
+                     "q_std": problem.qform_axis(3.0, sdims, origin=[q_hr[a, 3] + 0.75 for a in range(3)]),
+                     "neighborhood": 6.9}
+        images.append((hr, q_hr, th, emb, extra))
+    log(f"[bench] serving workload: {len(images)} images in {time.perf_counter() - t:.1f}s")
     reqs = []
     for i in range(n_req):
-        hr, qh, th, extra = images[i % len(images)]
-        reqs.append(serve.make_request(hr, qh, th, burnin=40, nsave=40, thin=1, L=L_STEPS, seed=1000 + i,
-                                       precision="fp32", **extra))
+        hr, qh, th, emb, extra = images[i % len(images)]
+        reqs.append(serve.make_request(hr, qh, th, burnin=burnin, nsave=nsave, thin=1, L=L_STEPS, seed=1000 + i,
+                                       precision="fp32", embed=emb, **extra))
+    return reqs
+
+
+def serving_leg(rates=(1.0, 2.0, 4.0), n_req=32, nsave=50, burnin=50, workers=0, seed=20261202):
+    """Mixed fits through the fit server, admitted by a 200 GB device-byte
+    budget (workers <= 0: no cap on concurrent fits besides the budget):
+    Poisson arrivals at each rate; p50/p99 submit-to-result latency, fits/s,
+    saved samples/s and post-burn-in grad evals/s per rate."""
+    from dualres_amd import serve
+    reqs = serving_requests(n_req, nsave, burnin, seed)
+    rng = np.random.default_rng(seed)
     out = []
     for rate in rates:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
@@ -213,9 +255,13 @@ def serving_leg(rates=(4.0, 8.0, 16.0), n_req=32, workers=4, seed=20261202, n_im
                 if dt > 0:
                     time.sleep(dt)
                 ids.append(srv.submit(r))
-                if (k + 1) % 100 == 0:
+                if (k + 1) % 50 == 0:
                     log(f"[bench] serving at {rate}/s: {k + 1} of {n_req} submitted")
-            res = [srv.wait(i, timeout=600) for i in ids]
+            res = []
+            for k, i in enumerate(ids):
+                res.append(srv.wait(i, timeout=1200))
+                if (k + 1) % 50 == 0:
+                    log(f"[bench] serving at {rate}/s: {k + 1} of {n_req} done")
             wall = time.perf_counter() - t0
         bad = [x["error"] for x in res if x["status"] != 0]
         if bad:
@@ -224,11 +270,14 @@ def serving_leg(rates=(4.0, 8.0, 16.0), n_req=32, workers=4, seed=20261202, n_im
         grads = sum(x["grad_evals"] for x in res)
         out.append({"arrival_rate_per_s": rate, "requests": n_req, "p50_s": round(float(np.percentile(lat, 50)), 4),
                     "p99_s": round(float(np.percentile(lat, 99)), 4), "fits_per_s": round(n_req / wall, 3),
+                    "saved_samples_per_s": round(n_req * nsave / wall, 1),
                     "post_burnin_grad_evals_per_s": round(grads / wall, 1)})
         log(f"[bench] serving at {rate}/s: p50 {out[-1]['p50_s']} s, p99 {out[-1]['p99_s']} s, "
             f"{out[-1]['fits_per_s']} fits/s")
-    return {"workers": workers, "budget_gb": 200, "workload": "48^3 / 64x80x64 @1.5mm, 50% dual, fp32, "
-            "40+40 iterations, L=25", "rates": out}
+    return {"workers": "budget" if workers <= 0 else workers, "budget_gb": 200,
+            "workload": "highres 48^3 / 64x80x64 / 96x112x96 / 128x160x128 @1.5mm, 50% dual (stdres 3mm), "
+                        f"theta (1, U(0.08,0.2), U(0.8,1.8)), fp32, {burnin} burn-in + {nsave} saved iterations, "
+                        "L=25", "rates": out}
 
 
 def batch_problems(n_chains=64, seed=7):
@@ -255,7 +304,7 @@ def batch_problems(n_chains=64, seed=7):
     return out
 
 
-def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, seed=900):
+def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, seed=900, tune=True):
     """Post-burn-in grad evals/s of all chains fitted in lockstep through one
     Batch (fit_batch: per chain dual averaging, Gibbs draws and saves; FP32
     gradients, FP64 energies), L = 25, in each batch mode; a chain's posterior
@@ -265,17 +314,25 @@ def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, s
     from dualres_amd.sampler import HMCParameters
     out = {"chains": len(probs), "workload": "64 dual-res images, highres U{32..112}^3 @1.5mm (stdres 3mm), "
            "theta (1, U(0.08,0.2), U(0.8,1.8)), 6.9mm neighbourhood, smooth embeddings, fp32 gradients, fp64 "
-           f"energies, L=25, {burnin} burn-in + {timed} timed iterations", "modes": {}}
+           f"energies, L=25, {burnin} burn-in + {timed} timed iterations", "autotuned": tune, "modes": {}}
     means = {}
+    import tempfile
+    cache = os.path.join(tempfile.mkdtemp(prefix="dualres_tune_"), "fft_tuning.json")
     for mode in modes:
         chains = []
+        t = time.perf_counter()
         for q in probs:
             c = Chain(q["ijk"], q["theta"], q["q_hr"], n_datasets=2, precision="fp32", state_mode="compat",
                       embed=q["embed"], seed=11)
             c.set_data(q["Yh"], q["Ys"], q["W"])
             c.add_data_to_mu()
             c.set_output_indices(q["ijk"])
+            if tune:
+                c.set_sigma_sq_inv([1.0, 1.0])
+                c.autotune(cache)   # timed once per configuration, then read from the cache
             chains.append(c)
+        if tune:
+            log(f"[bench] batch {mode}: chains set up and tuned in {time.perf_counter() - t:.1f}s")
         hmcs = [HMCParameters(burnin=burnin, n_save=timed, thin=1, integrator_steps=L_STEPS,
                               starting_eps=1.0 / L_STEPS) for _ in probs]
 
@@ -297,6 +354,70 @@ def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, s
     best = max(out["modes"], key=lambda m: out["modes"][m]["post_burnin_grad_evals_per_s"])
     out["best_mode"] = best
     out["post_burnin_grad_evals_per_s"] = out["modes"][best]["post_burnin_grad_evals_per_s"]
+    return out
+
+
+def mce_leg(n_images=32, seed=5, nbins=64, maxdist=20.0):
+    """RBF parameter estimation on a batch of synthetic 128x160x128 images at
+    1 mm: each a GP draw with (tau^2, psi, nu) ~ U(0.5,1.5) x U(0.05,0.3) x
+    U(0.5,1.9) on the ellipsoid mask plus N(0,1) noise.  The draws come from
+    the engine itself: a chain's initial state is a circulant-embedded GP
+    draw scaled by 1/10 (MultiResParameters.h:743-759).  Per image, timed:
+    the binned empirical covariance (nbins bins to maxdist mm, FP64 bin sums)
+    and its constrained minimum-contrast fit (psi <= nu), and the per-offset
+    path estimate_rbf takes (25,348 offsets, then the same fit).  Parameter
+    errors: binned against per-offset estimates and both against the truth."""
+    from dualres_amd import Chain, problem
+    from dualres_amd.mce import mce_binned, mce_summary, rbf_fit
+    dims, vox = (128, 160, 128), 1.0
+    mask = problem.ellipsoid_mask(dims, 0.45)
+    ijk = problem.nonzero_ijk(mask.astype(np.float64))
+    lo = ijk.min(0)
+    q = problem.qform_axis(vox, dims)
+    flat = np.flatnonzero(mask.ravel())
+    rng = np.random.default_rng(seed)
+    tb, tp, thetas = [], [], []
+    t0 = time.perf_counter()
+    for k in range(n_images):
+        th = (float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.05, 0.3)), float(rng.uniform(0.5, 1.9)))
+        c = Chain(ijk - lo, th, q, n_datasets=1, precision="fp32", state_mode="real", embed=EMBED, seed=7000 + k)
+        c.set_data(np.zeros(ijk.shape[0]))
+        draw = 10.0 * c.mu()
+        c.close()
+        img = np.zeros(mask.size)
+        img[flat] = draw + rng.standard_normal(flat.size)
+        img = img.reshape(mask.shape)
+        t = time.perf_counter()
+        bc, bd, bn = mce_binned(img, q, nbins=nbins, maxdist=maxdist)
+        theta_b, rc_b = rbf_fit(bc, bd, bn, constrained=True)
+        tb.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        cov, dist, npairs = mce_summary(img, q)
+        theta_p, rc_p = rbf_fit(cov, dist, npairs, constrained=True)
+        tp.append(time.perf_counter() - t)
+        thetas.append((th, theta_b.tolist(), theta_p.tolist(), rc_b, rc_p))
+        if (k + 1) % 8 == 0:
+            log(f"[bench] estimation: {k + 1} of {n_images} images")
+    wall = time.perf_counter() - t0
+    T = np.array([t[0] for t in thetas])
+    B = np.array([t[1] for t in thetas])
+    P = np.array([t[2] for t in thetas])
+
+    def med_rel(a, b):
+        return [round(float(np.median(np.abs(a[:, j] - b[:, j]) / np.abs(b[:, j]))), 4) for j in range(3)]
+    out = {"images": n_images, "workload": f"{dims[0]}x{dims[1]}x{dims[2]} @1mm ellipsoid-masked GP draws + N(0,1), "
+           f"theta ~ U(0.5,1.5) x U(0.05,0.3) x U(0.5,1.9); {nbins} bins to {maxdist:g} mm; constrained fit",
+           "binned": {"images_per_s": round(n_images / sum(tb), 3), "p50_ms": round(1e3 * float(np.median(tb)), 2),
+                      "p99_ms": round(1e3 * float(np.percentile(tb, 99)), 2)},
+           "per_offset": {"images_per_s": round(n_images / sum(tp), 3),
+                          "p50_ms": round(1e3 * float(np.median(tp)), 2)},
+           "median_rel_error": {"binned_vs_per_offset": med_rel(B, P), "binned_vs_truth": med_rel(B, T),
+                                "per_offset_vs_truth": med_rel(P, T)},
+           "fit_status_ok": int(sum(1 for t in thetas if t[3] == 0 and t[4] == 0)),
+           "wall_s_including_draws": round(wall, 1)}
+    log(f"[bench] estimation: binned {out['binned']['images_per_s']} images/s, per-offset "
+        f"{out['per_offset']['images_per_s']} images/s; median rel. error binned vs per-offset "
+        f"{out['median_rel_error']['binned_vs_per_offset']}")
     return out
 
 
@@ -396,12 +517,17 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-serve", action="store_true", help="skip the fit-server latency leg")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests per arrival rate in the serving leg")
-    ap.add_argument("--serve-rates", default="4,8,16", help="Poisson arrival rates (fits/s) of the serving leg")
+    ap.add_argument("--serve-rates", default="1,2,4", help="Poisson arrival rates (fits/s) of the serving leg")
+    ap.add_argument("--serve-nsave", type=int, default=50, help="saved (and burn-in) iterations per served fit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device-to-device copy (reference for HBM counters)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the --precision complex-state chain (for profiler runs)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched-chains leg and print its JSON")
+    ap.add_argument("--mce-only", action="store_true", help="run only the parameter-estimation leg and print its JSON")
+    ap.add_argument("--mce-images", type=int, default=32, help="images in the parameter-estimation leg")
+    ap.add_argument("--batch-modes", default="graph,streams,eager", help="batch modes to time, in order")
+    ap.add_argument("--no-tune", action="store_true", help="batched chains keep the built-in FFT pass shapes")
     ap.add_argument("--cpu-study", action="store_true",
                     help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
                          "medians of 3 iterations) and print its JSON")
@@ -414,14 +540,18 @@ def main():
         sys.exit(2)
     if a.serve_only:
         print(json.dumps(serving_leg(rates=tuple(float(x) for x in a.serve_rates.split(",")),
-                                     n_req=a.serve_requests)), flush=True)
+                                     n_req=a.serve_requests, nsave=a.serve_nsave, burnin=a.serve_nsave)), flush=True)
         return
     p = build_problem()
     if a.cpu_study:
         print(json.dumps(cpu_study(p, eps=0.54)), flush=True)
         return
+    if a.mce_only:
+        print(json.dumps(mce_leg(a.mce_images)), flush=True)
+        return
     if a.batch_only:
-        print(json.dumps(batch_leg(batch_problems())), flush=True)
+        print(json.dumps(batch_leg(batch_problems(), modes=tuple(a.batch_modes.split(",")),
+                                   tune=not a.no_tune)), flush=True)
         return
     results = {}
     main_chain = None
@@ -457,8 +587,11 @@ def main():
     if a.calib_copy:
         calibration_copy()
     serving = None if a.main_only or a.no_serve else serving_leg(
-        rates=tuple(float(x) for x in a.serve_rates.split(",")), n_req=a.serve_requests)
-    batch = None if a.main_only or a.no_serve else batch_leg(batch_problems())
+        rates=tuple(float(x) for x in a.serve_rates.split(",")), n_req=a.serve_requests, nsave=a.serve_nsave,
+        burnin=a.serve_nsave)
+    batch = None if a.main_only or a.no_serve else batch_leg(batch_problems(), modes=tuple(a.batch_modes.split(",")),
+                                                             tune=not a.no_tune)
+    estimation = None if a.main_only or a.no_serve else mce_leg(a.mce_images)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(p, eps=results[a.precision]["eps"])
@@ -494,6 +627,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(byt), "launch_ms": round(ks[dom], 5)},
         "serving_latency": serving,
         "batch_throughput": batch,
+        "parameter_estimation": estimation,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

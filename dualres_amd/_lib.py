@@ -88,6 +88,9 @@ SIGNATURES = {
     "dr_mu_data_sumsq": [_P, _D],
     "dr_update": [_P, C.c_double, C.c_int, C.c_double, C.POINTER(dr_update_result)],
     "dr_update_with_noise": [_P, C.c_double, C.c_int, C.c_double, _D, _D, C.POINTER(dr_update_result)],
+    "dr_autotune": [_P, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "dr_set_tuning": [_P, C.POINTER(C.c_int)],
+    "dr_get_tuning": [_P, C.POINTER(C.c_int)],
     "dr_update_captured": [_P, C.c_double, C.c_int, C.c_double, C.POINTER(dr_update_result)],
     "dr_batch_create": [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P)],
     "dr_batch_update": [_P, _D, C.POINTER(C.c_int), _D, C.POINTER(dr_update_result)],

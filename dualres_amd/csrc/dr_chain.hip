@@ -15,13 +15,18 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
+
+#include <unistd.h>
 
 #include "dr_kernels.hpp"
 
@@ -185,6 +190,10 @@ struct ChainBase {
   virtual hipGraph_t capture_iteration(int L) = 0;
   virtual void enqueue_iteration(double eps, int L, hipStream_t s) = 0;
   virtual void enqueue_eager(double eps, int L) = 0;
+  // autotuned pass shapes: 5 tile-line exponents, 3 radix orders (see Chain)
+  virtual void set_tuning(const int* prm) = 0;
+  virtual void get_tuning(int* prm) const = 0;
+  virtual void autotune(int* prm) = 0;
   virtual void finish_iteration(const double* h, double u, dr_update_result* res, bool copy, hipStream_t s) = 0;
 
   long region_index(int x, int y, int z) const { return x + (long)reg[0] * (y + (long)reg[1] * z); }
@@ -258,6 +267,18 @@ struct Chain : public ChainBase {
     }
   }
 
+  // Autotuned pass shapes (dr_autotune): tile lines (log2) of the z, inverse
+  // y, inverse x, forward x and forward y passes of a leapfrog step, -1 for
+  // pick_tbs's rule, capped at what the tile and LDS allow; and the radix
+  // order of each axis's runtime stages (make_axis).
+  enum { P_Z = 0, P_YI, P_XI, P_XF, P_YF, NPASS };
+  int tune_tbs[NPASS] = {-1, -1, -1, -1, -1};
+  int tune_order[3] = {0, 0, 0};
+  int tuned_tbs(int pid, int rule, int cap) const {
+    if (pid < 0 || tune_tbs[pid] < 0) return rule;
+    return std::min(tune_tbs[pid], cap);
+  }
+
   ~Chain() override {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
@@ -278,11 +299,12 @@ struct Chain : public ChainBase {
 
   // transform along z (a=2) or y (a=1) over lines with contiguous batch NB
   template <typename U, bool INV, class Pro, class Epi>
-  void strided(int a, int n_outer, int NB, int k_in, int k_out, Pro pro, Epi epi, int slot = -1) {
+  void strided(int a, int n_outer, int NB, int k_in, int k_out, Pro pro, Epi epi, int slot = -1, int pid = -1) {
     const AxisFFT<U> ad = axis_dev<U>(a);
     DR_REQUIRE(k_in >= 1 && k_in <= ad.M && k_out >= 1 && k_out <= ad.M && NB >= 1 && n_outer >= 1,
                DR_ELOGIC, "strided pass: bad extents");
-    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, kTileStrided);
+    const int tbs = tuned_tbs(pid, pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, NB, kTileStrided),
+                              pick_tbs(ad.M, ad.M, sizeof(cx<U>), false, 1 << 20, kTileStrided));
     const int TB = 1 << tbs;
     dim3 grid(cdiv(NB, TB), n_outer);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<U>);
@@ -297,10 +319,11 @@ struct Chain : public ChainBase {
     mark();
   }
   template <typename U, class Pro, class Epi>
-  void xr2c(int nrows, Pro pro, Epi epi, int slot = -1) {
+  void xr2c(int nrows, Pro pro, Epi epi, int slot = -1, int pid = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileX);
+    const int tbs = tuned_tbs(pid, pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, xlines(nrows, Mh), kTileX),
+                              pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, 1 << 20, kTileX));
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -315,13 +338,14 @@ struct Chain : public ChainBase {
     mark();
   }
   template <typename U, class Pro, class Epi>
-  void xc2r(int nrows, Pro pro, Epi epi, int slot = -1) {
+  void xc2r(int nrows, Pro pro, Epi epi, int slot = -1, int pid = -1) {
     const AxisFFT<U> ad = axis_dev<U>(0);
     const int Mh = ad.M;
     // twice the forward pass's rows per tile: at 1 mm (16,704 rows of 128)
     // 16-row tiles ran the inverse pass in 13.8 us against 16.2 us for 8 rows,
     // while the forward pass with its residual prologue was faster at 8
-    const int tbs = pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, 2 * xlines(nrows, Mh), kTileX);
+    const int tbs = tuned_tbs(pid, pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, 2 * xlines(nrows, Mh), kTileX),
+                              pick_tbs(Mh, Mh + 1, sizeof(cx<U>), true, 1 << 20, kTileX));
     const int TB = 1 << tbs;
     const int grid = cdiv(nrows, TB);
     const size_t lds = (size_t)(Mh + 1) * (TB + 1) * sizeof(cx<U>);
@@ -1329,20 +1353,22 @@ struct Chain : public ChainBase {
   // inverse y and x passes from T1 (z < n2) to the real region field R
   void c2r_yx(T* R) {
     strided<T, true>(1, reg[2], H0, M[1], reg[1], LoadS<T>{{}, T1.p, (long)M[1] * H0, H0},
-                     StoreS<T>{{}, T2.p, (long)reg[1] * H0, H0});
-    xc2r<T>(reg[1] * reg[2], LoadRowC<T>{{}, T2.p, H0}, StoreRowR<T>{{}, R, reg[0], (T)(1.0 / (double)N)});
+                     StoreS<T>{{}, T2.p, (long)reg[1] * H0, H0}, -1, P_YI);
+    xc2r<T>(reg[1] * reg[2], LoadRowC<T>{{}, T2.p, H0}, StoreRowR<T>{{}, R, reg[0], (T)(1.0 / (double)N)}, -1,
+            P_XI);
   }
   // forward x and y passes of the scattered residual into T1 (z < n2)
   void r2c_xy(LoadResidual<T> pro, int pro_slot) {
-    xr2c<T>(reg[1] * reg[2], pro, StoreRowC<T>{{}, T2.p, H0}, pro_slot);
+    xr2c<T>(reg[1] * reg[2], pro, StoreRowC<T>{{}, T2.p, H0}, pro_slot, P_XF);
     strided<T, false>(1, reg[2], H0, reg[1], M[1], LoadS<T>{{}, T2.p, (long)reg[1] * H0, H0},
-                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
+                      StoreS<T>{{}, T1.p, (long)M[1] * H0, H0}, -1, P_YF);
   }
   // forward z pass, kick + drift, inverse z pass, in place on T1
   void zfused(MidLeapfrog<T> mid, int mid_slot) {
     const AxisFFT<T>& ad = axis_dev<T>(2);
     const int NB = M[1] * H0;
-    const int tbs = pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, NB, kTileStrided);
+    const int tbs = tuned_tbs(P_Z, pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, NB, kTileStrided),
+                              pick_tbs(ad.M, ad.M, sizeof(cx<T>), false, 1 << 20, kTileStrided));
     const int TB = 1 << tbs;
     const int grid = cdiv(NB, TB);
     const size_t lds = (size_t)ad.M * TB * sizeof(cx<T>);
@@ -1597,6 +1623,96 @@ struct Chain : public ChainBase {
     DR_HIP(hipMemcpyAsync(dsc, hsc, sizeof(StepScalars<T>), hipMemcpyHostToDevice, s));
     DR_HIP(hipGraphLaunch(e, s));
   }
+  // ------------------------------------------------------------ autotuning
+  static bool ct_length(int n) {
+    return n == 32 || n == 64 || n == 128 || n == 160 || n == 256 || n == 320 || n == 512;
+  }
+  void set_tuning(const int* prm) override {
+    for (int i = 0; i < NPASS; ++i) tune_tbs[i] = prm[i] < 0 ? -1 : prm[i];
+    for (int a = 0; a < 3; ++a) {
+      const int o = prm[NPASS + a] < 0 ? 0 : prm[NPASS + a];
+      DR_REQUIRE(o <= 2, DR_EDOMAIN, "radix order must be 0, 1 or 2");
+      tune_order[a] = o;
+      const int flen = a == 0 ? M[0] / 2 : M[a], tws = a == 0 ? 2 : 1;
+      if (std::is_same<T, float>::value) adf[a] = make_axis<float>(flen, twf[a].p, tws, o);
+      else add[a] = make_axis<double>(flen, twd[a].p, tws, o);
+    }
+    drop_graphs();
+    stats_valid = false;
+  }
+  void get_tuning(int* prm) const override {
+    for (int i = 0; i < NPASS; ++i) prm[i] = tune_tbs[i];
+    for (int a = 0; a < 3; ++a) prm[NPASS + a] = tune_order[a];
+  }
+  // device ms of a 2-step leapfrog (3 gradients) from the current state,
+  // best of 3 after one warm-up; the state A is left as it was
+  double time_leapfrog() {
+    auto run = [&] {
+      clear_slots();
+      leapfrog_core(1e-3, 2);
+    };
+    run();
+    double best = 1e30;
+    for (int r = 0; r < 3; ++r) {
+      DR_HIP(hipEventRecord(ev[2], stream));
+      run();
+      DR_HIP(hipEventRecord(ev[3], stream));
+      DR_HIP(hipEventSynchronize(ev[3]));
+      float ms = 0.f;
+      DR_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+      best = std::min(best, (double)ms);
+    }
+    return best;
+  }
+  // coordinate search: each pass's tile lines over every size the tile and
+  // LDS allow, then each axis's radix order where the runtime stages run
+  void autotune(int* prm) override {
+    DR_REQUIRE(data_set, DR_ELOGIC, "autotune: data not set");
+    const bool prof0 = prof;
+    prof = false;
+    rec = -1;
+    int cur[NPASS + 3];
+    get_tuning(cur);
+    const int Mh = M[0] / 2;
+    const int capz = pick_tbs(M[2], M[2], sizeof(cx<T>), false, 1 << 20, kTileStrided);
+    const int capy = pick_tbs(M[1], M[1], sizeof(cx<T>), false, 1 << 20, kTileStrided);
+    const int capx = pick_tbs(Mh, Mh + 1, sizeof(cx<T>), true, 1 << 20, kTileX);
+    const int cap[NPASS] = {capz, capy, capx, capx, capy};
+    set_tuning(cur);
+    double best = time_leapfrog();
+    for (int pid = 0; pid < NPASS; ++pid) {
+      int arg = cur[pid];
+      for (int v = 0; v <= cap[pid]; ++v) {
+        cur[pid] = v;
+        set_tuning(cur);
+        const double t = time_leapfrog();
+        if (t < best) {
+          best = t;
+          arg = v;
+        }
+      }
+      cur[pid] = arg;
+    }
+    const int len[3] = {Mh, M[1], M[2]};
+    for (int a = 0; a < 3; ++a) {
+      if (ct_length(len[a])) continue;
+      int arg = cur[NPASS + a];
+      for (int o = 0; o <= 2; ++o) {
+        cur[NPASS + a] = o;
+        set_tuning(cur);
+        const double t = time_leapfrog();
+        if (t < best) {
+          best = t;
+          arg = o;
+        }
+      }
+      cur[NPASS + a] = arg;
+    }
+    set_tuning(cur);
+    prof = prof0;
+    for (int i = 0; i < NPASS + 3; ++i) prm[i] = cur[i];
+  }
+
   // the eager launches of one iteration (the update's kernels, on the
   // chain's stream, no wait)
   void enqueue_eager(double eps, int L) override {
@@ -2023,6 +2139,107 @@ int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept, const 
   return guarded([&] {
     DR_REQUIRE(z_re && (z_im || !c->impl->compat()), DR_ELOGIC, "dr_update_with_noise: null noise");
     c->impl->update(eps, L, u_accept, z_re, z_im, res);
+  });
+}
+
+// tuning cache: {"key": [8 ints], ...}, one entry per line
+static std::map<std::string, std::vector<int>> read_tune_cache(const std::string& path) {
+  std::map<std::string, std::vector<int>> m;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    const size_t q1 = line.find('"');
+    if (q1 == std::string::npos) continue;
+    const size_t q2 = line.find('"', q1 + 1);
+    if (q2 == std::string::npos) continue;
+    const size_t b1 = line.find('[', q2), b2 = line.find(']', b1 == std::string::npos ? q2 : b1);
+    if (b1 == std::string::npos || b2 == std::string::npos) continue;
+    std::vector<int> v;
+    std::stringstream ss(line.substr(b1 + 1, b2 - b1 - 1));
+    std::string tok;
+    try {
+      while (std::getline(ss, tok, ',')) v.push_back(std::stoi(tok));
+    } catch (...) {
+      continue;
+    }
+    if (v.size() == 8) m[line.substr(q1 + 1, q2 - q1 - 1)] = v;
+  }
+  return m;
+}
+static void write_tune_cache(const std::string& path, const std::map<std::string, std::vector<int>>& m) {
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+  {
+    std::ofstream f(tmp);
+    DR_REQUIRE(f.is_open(), DR_ERUNTIME, "autotune: cannot write the cache " + path);
+    f << "{\n";
+    size_t k = 0;
+    for (const auto& kv : m) {
+      f << "  \"" << kv.first << "\": [";
+      for (size_t i = 0; i < kv.second.size(); ++i) f << (i ? ", " : "") << kv.second[i];
+      f << "]" << (++k < m.size() ? "," : "") << "\n";
+    }
+    f << "}\n";
+  }
+  DR_REQUIRE(std::rename(tmp.c_str(), path.c_str()) == 0, DR_ERUNTIME, "autotune: cannot replace the cache " + path);
+}
+// cache key: device architecture, precision, state mode, datasets, torus
+// and image region
+static std::string tune_key(const drh::ChainBase& c) {
+  hipDeviceProp_t prop;
+  DR_HIP(hipGetDeviceProperties(&prop, c.cfg.device));
+  std::string arch = prop.gcnArchName;
+  arch = arch.substr(0, arch.find(':'));
+  std::ostringstream k;
+  k << arch << "|" << (c.cfg.precision == DR_FP64 ? "fp64" : "fp32") << "|"
+    << (c.cfg.state_mode == DR_STATE_COMPAT ? "compat" : "real") << "|" << c.cfg.n_datasets << "|" << c.M[0] << "x"
+    << c.M[1] << "x" << c.M[2] << "|" << c.reg[0] << "x" << c.reg[1] << "x" << c.reg[2];
+  return k.str();
+}
+
+int dr_autotune(dr_chain* c, const char* cache_path, int* params, int* from_cache) {
+  return guarded([&] {
+    DR_REQUIRE(c && params, DR_ELOGIC, "dr_autotune: null argument");
+    auto& k = *c->impl;
+    DR_HIP(hipSetDevice(k.cfg.device));
+    DR_HIP(hipStreamSynchronize(k.stream));
+    int prm[8];
+    bool hit = false;
+    std::string key;
+    if (cache_path && *cache_path) {
+      key = tune_key(k);
+      const auto m = read_tune_cache(cache_path);
+      const auto f = m.find(key);
+      if (f != m.end()) {
+        std::copy(f->second.begin(), f->second.end(), prm);
+        k.set_tuning(prm);
+        hit = true;
+      }
+    }
+    if (!hit) {
+      k.autotune(prm);
+      if (cache_path && *cache_path) {
+        auto m = read_tune_cache(cache_path);
+        m[key] = std::vector<int>(prm, prm + 8);
+        write_tune_cache(cache_path, m);
+      }
+    }
+    std::copy(prm, prm + 8, params);
+    if (from_cache) *from_cache = hit ? 1 : 0;
+  });
+}
+
+int dr_set_tuning(dr_chain* c, const int* params) {
+  return guarded([&] {
+    DR_REQUIRE(c && params, DR_ELOGIC, "dr_set_tuning: null argument");
+    DR_HIP(hipStreamSynchronize(c->impl->stream));
+    c->impl->set_tuning(params);
+  });
+}
+
+int dr_get_tuning(dr_chain* c, int* params) {
+  return guarded([&] {
+    DR_REQUIRE(c && params, DR_ELOGIC, "dr_get_tuning: null argument");
+    c->impl->get_tuning(params);
   });
 }
 

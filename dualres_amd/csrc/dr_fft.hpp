@@ -651,8 +651,10 @@ std::vector<cx<T>> twiddles(int M) {
 
 // Stage radices in factorize()'s order, into the descriptor's fixed array
 // (no heap use, so the allocation-free entry points can build plans).
+// order 1 runs the odd radices (3, 5, 7) first, order 2 reverses the default
+// sequence; the compile-time pipelines ignore it.
 template <typename T>
-AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws) {
+AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws, int order = 0) {
   AxisFFT<T> a{};
   a.M = fft_len;
   if (fft_len < 1) throw Error(DR_EDOMAIN, "FFT length must be positive");
@@ -670,6 +672,20 @@ AxisFFT<T> make_axis(int fft_len, const cx<T>* tw, int tws) {
   for (int r : {3, 5, 7})
     while (m % r == 0) take(r);
   if (m != 1) throw Error(DR_EDOMAIN, "FFT length " + std::to_string(fft_len) + " has a prime factor > 7");
+  if (order == 1) {
+    int r[kMaxStages], n = 0;
+    for (int i = 0; i < a.nst; ++i)
+      if (a.radix[i] % 2) r[n++] = a.radix[i];
+    for (int i = 0; i < a.nst; ++i)
+      if (a.radix[i] % 2 == 0) r[n++] = a.radix[i];
+    for (int i = 0; i < a.nst; ++i) a.radix[i] = r[i];
+  } else if (order == 2) {
+    for (int i = 0, j = a.nst - 1; i < j; ++i, --j) {
+      const int t = a.radix[i];
+      a.radix[i] = a.radix[j];
+      a.radix[j] = t;
+    }
+  }
   a.tw = tw;
   a.tws = tws;
   return a;

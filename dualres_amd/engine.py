@@ -200,6 +200,24 @@ class Chain:
                                                    L.dptr(zr), L.dptr(zi), C.byref(res)))
         return res.as_dict()
 
+    def autotune(self, cache=None):
+        """Pick the FFT pass shapes by timing (or from the JSON cache file);
+        returns (params, from_cache)."""
+        prm = (C.c_int * 8)()
+        hit = C.c_int()
+        path = None if cache is None else str(cache).encode()
+        L.check(self._lib.dr_autotune(self._h, path, prm, C.byref(hit)))
+        return list(prm), bool(hit.value)
+
+    def set_tuning(self, params):
+        prm = (C.c_int * 8)(*[int(v) for v in params])
+        L.check(self._lib.dr_set_tuning(self._h, prm))
+
+    def tuning(self):
+        prm = (C.c_int * 8)()
+        L.check(self._lib.dr_get_tuning(self._h, prm))
+        return list(prm)
+
     def update_captured(self, eps, L_steps, u_accept):
         """update() through the chain's captured HIP graph for this leapfrog length."""
         res = L.dr_update_result()

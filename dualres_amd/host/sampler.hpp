@@ -217,6 +217,18 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
   out << std::setprecision(6);
   if (nonpos > 0) out << "\t<Covariance function is numerically not positive definite>" << std::endl;
   chain.set_data(data.Yh, &data.Ys, &data.W);
+  if (data.n_datasets == 2) {
+    // which device layout W took: the interior rows as one stencil with packed
+    // boundary rows, or plain CSR when the stencil layout does not apply
+    int64_t st[11];
+    check(dr_w_stats(chain.handle(), st));
+    if (st[0])
+      out << "W layout: stencil for " << st[1] << " interior rows (" << st[2] << " entries), packed entries for "
+          << st[3] << " boundary rows" << std::endl;
+    else
+      out << "W layout: plain CSR (" << st[5] << " nonzeros; the stencil layout does not apply to this W)"
+          << std::endl;
+  }
   // _initialize_sigma (MultiResParameters.h:764-778), after the mu draw
   std::vector<double> sig(data.n_datasets);
   {

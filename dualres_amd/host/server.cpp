@@ -133,8 +133,12 @@ ServeResult run_fit(const ServeRequest& r, int device) {
 
 Server::Server(int64_t budget_bytes, int workers, int device, bool backfill)
     : budget_(budget_bytes), device_(device), backfill_(backfill) {
-  if (workers < 1) throw std::domain_error("Server: at least one worker");
-  for (int i = 0; i < workers; ++i) threads_.emplace_back([this] { worker(); });
+  if (workers <= 0) {
+    uncapped_ = true;
+    threads_.emplace_back([this] { dispatcher(); });
+  } else {
+    for (int i = 0; i < workers; ++i) threads_.emplace_back([this] { worker(); });
+  }
 }
 
 Server::~Server() {
@@ -144,6 +148,7 @@ Server::~Server() {
   }
   cv_work_.notify_all();
   for (auto& t : threads_) t.join();
+  for (auto& f : fits_) f.first.join();
 }
 
 int64_t Server::submit(ServeRequest r) {
@@ -160,8 +165,43 @@ int64_t Server::submit(ServeRequest r) {
     return id;
   }
   queue_.push_back(Job{id, std::move(r), bytes, now_s()});
-  cv_work_.notify_one();
+  cv_work_.notify_all();
   return id;
+}
+
+bool Server::take(Job* job) {
+  auto it = queue_.end();
+  for (auto q = queue_.begin(); q != queue_.end(); ++q) {
+    if (in_use_ + q->bytes <= budget_) {
+      it = q;
+      break;
+    }
+    if (!backfill_) break;  // strict FIFO: wait for the head
+  }
+  if (it == queue_.end()) return false;
+  *job = std::move(*it);
+  queue_.erase(it);
+  in_use_ += job->bytes;
+  ++running_;
+  return true;
+}
+
+void Server::run_job(Job job) {
+  const double t0 = now_s();
+  ServeResult res = run_fit(job.req, device_);
+  const double t1 = now_s();
+  res.queue_s = t0 - job.t_submit;
+  res.run_s = t1 - t0;
+  res.total_s = t1 - job.t_submit;
+  res.device_bytes = job.bytes;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    in_use_ -= job.bytes;
+    --running_;
+    done_[job.id] = std::move(res);
+  }
+  cv_done_.notify_all();
+  cv_work_.notify_all();
 }
 
 void Server::worker() {
@@ -171,39 +211,39 @@ void Server::worker() {
       std::unique_lock<std::mutex> lk(mu_);
       for (;;) {
         if (stop_ && queue_.empty()) return;
-        auto it = queue_.end();
-        for (auto q = queue_.begin(); q != queue_.end(); ++q) {
-          if (in_use_ + q->bytes <= budget_) {
-            it = q;
-            break;
-          }
-          if (!backfill_) break;  // strict FIFO: wait for the head
-        }
-        if (it != queue_.end()) {
-          job = std::move(*it);
-          queue_.erase(it);
-          in_use_ += job.bytes;
-          ++running_;
-          break;
-        }
+        if (take(&job)) break;
         cv_work_.wait(lk);
       }
     }
-    const double t0 = now_s();
-    ServeResult res = run_fit(job.req, device_);
-    const double t1 = now_s();
-    res.queue_s = t0 - job.t_submit;
-    res.run_s = t1 - t0;
-    res.total_s = t1 - job.t_submit;
-    res.device_bytes = job.bytes;
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      in_use_ -= job.bytes;
-      --running_;
-      done_[job.id] = std::move(res);
+    run_job(std::move(job));
+  }
+}
+
+// uncapped server: admit every job the budget allows, each on a new thread;
+// threads of finished fits are joined as the dispatcher goes
+void Server::dispatcher() {
+  std::unique_lock<std::mutex> lk(mu_);
+  for (;;) {
+    for (auto it = fits_.begin(); it != fits_.end();) {
+      if (it->second->load()) {
+        it->first.join();
+        it = fits_.erase(it);
+      } else {
+        ++it;
+      }
     }
-    cv_done_.notify_all();
-    cv_work_.notify_all();
+    if (stop_ && queue_.empty()) return;
+    Job job;
+    if (take(&job)) {
+      auto flag = std::make_shared<std::atomic<bool>>(false);
+      fits_.emplace_back(std::thread([this, flag, j = std::move(job)]() mutable {
+                           run_job(std::move(j));
+                           flag->store(true);
+                         }),
+                         flag);
+      continue;
+    }
+    cv_work_.wait(lk);
   }
 }
 

@@ -5,16 +5,20 @@
 // build's scratch for a dual-resolution fit) fit under the budget next to
 // the fits already running; with backfill on, a later request that fits may
 // start while a larger one at the head waits.  Admitted fits run to
-// completion on worker threads, each with its own chain (own HIP stream) and
-// its own host generator seeded from the request, so a fit's result does not
-// depend on what else runs beside it.
+// completion on their own threads, each with its own chain (own HIP stream)
+// and its own host generator seeded from the request, so a fit's result does
+// not depend on what else runs beside it.  With workers > 0 at most that many
+// fits run at once; with workers <= 0 the device-byte budget alone limits
+// how many run together (a thread is started per admitted fit).
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -69,7 +73,13 @@ class Server {
     double t_submit;
   };
   void worker();
+  void dispatcher();
+  bool take(Job* job);  // under mu_: pop the first admissible job
+  void run_job(Job job);
   int64_t budget_;
+  bool uncapped_ = false;
+  // per-fit threads of an uncapped server, with a flag set when each ends
+  std::deque<std::pair<std::thread, std::shared_ptr<std::atomic<bool>>>> fits_;
   int device_;
   bool backfill_;
   bool stop_ = false;

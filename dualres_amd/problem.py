@@ -111,7 +111,7 @@ def single_problem(dims, vox, theta, seed_gp=20261202, seed_noise=1, frac=0.45, 
 def dual_problem(dims, vox, theta, seed_gp=20261202, seed_noise=(1, 2), frac=0.45, noise_h=1.0, noise_s=0.5):
     """Highres + 2x2x2 block-mean stdres images of the same GP draw."""
     p = single_problem(dims, vox, theta, seed_gp, seed_noise[0], frac, noise_h)
-    vox = float(vox)
+    vox = np.broadcast_to(np.asarray(vox, dtype=np.float64), (3,))
     sdims = tuple(d // 2 for d in dims)
     f = p["truth"]
     s = block_means(f, 2)
@@ -121,7 +121,7 @@ def dual_problem(dims, vox, theta, seed_gp=20261202, seed_noise=(1, 2), frac=0.4
     s = nudge_zeros(s, sm)
     std = np.where(sm, s, 0.0)
     q_hr = p["q_hr"]
-    q_std = qform_axis(2 * vox, sdims, origin=[q_hr[a, 3] + 0.5 * vox for a in range(3)])
+    q_std = qform_axis(2 * vox, sdims, origin=[q_hr[a, 3] + 0.5 * vox[a] for a in range(3)])
     flat = np.flatnonzero(present(std).ravel())
     p.update({"std": std, "smask": sm, "q_std": q_std, "sdims": sdims, "Ys": std.ravel()[flat]})
     return p

@@ -177,6 +177,23 @@ int dr_update_with_noise(dr_chain* c, double eps, int L, double u_accept,
  * (its L-step loop is MultiResParameters.h:691-700). */
 int dr_update_captured(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res);
 
+/* FFT pass shapes picked by timing (an autotuner in the role of FFTW's plan
+ * search and wisdom, sized_fft.h:259-269 and src/preplan_fft.cpp:84-156).
+ * params[8]: the tile lines (log2) of the fused z pass, the inverse y and x
+ * passes and the forward x (residual) and y passes of a leapfrog step, then
+ * the radix order of the x, y and z runtime stages (0 largest radix first,
+ * 1 odd radices first, 2 smallest first; lengths with a compile-time stage
+ * pipeline keep theirs).  dr_autotune times each choice on the chain's own
+ * buffers (a 2-step leapfrog from the current state, which it leaves as it
+ * was), unless cache_path names a JSON file that holds an entry for this
+ * device architecture, precision, state mode, dataset count, torus and image
+ * region; a timed result is added to that file.  The tile choice changes
+ * only the blocking of the energy reductions, the radix order the rounding
+ * of the transforms.  dr_set_tuning applies params (-1: built-in rule). */
+int dr_autotune(dr_chain* c, const char* cache_path, int* params, int* from_cache);
+int dr_set_tuning(dr_chain* c, const int* params);
+int dr_get_tuning(dr_chain* c, int* params);
+
 /* --- batches: many chains of uneven size advanced together --------------- */
 enum dr_batch_mode {
   DR_BATCH_GRAPH = 0,   /* one graph per iteration: every chain's captured

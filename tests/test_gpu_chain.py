@@ -481,3 +481,26 @@ def test_kriging_pattern_flat_offset_wrap(gpu_available):
     W = kriging_build(hr, q_hr, std, q_std, theta, 4.6)
     assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
     assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
+
+
+def test_kriging_indefinite_systems_tau2_above_one(gpu_available):
+    """tau^2 = 1.5 with the reference's unit diagonal (kriging_matrix.h:118)
+    makes K indefinite, so the device Cholesky meets a nonpositive pivot and
+    those systems are solved again with partial pivoting: pattern bit-exact
+    and weights within 1e-6 of the oracle's pivoted float64 solves
+    (kriging_matrix.h:309-313 solves the same systems by column-pivoted QR)."""
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
+    from oracle import kriging
+    theta = (1.5, 0.132, 1.5)
+    p = problem.dual_problem((14, 12, 10), 2.0, theta, seed_gp=19)
+    radius = 2.3 * 2.0
+    P = kriging.neighborhood_perturbation(p["q_hr"], radius)
+    K = kriging.perturbation_kernel_distances(P, p["q_hr"], theta).astype(np.float64)
+    assert np.linalg.eigvalsh(K).min() < 0          # indefinite
+    rp, cols, vals, nrow, ncol, st = kriging.get_sparse_kriging_matrix_data(
+        p["hr"], p["std"], p["q_hr"], p["q_std"], theta, radius)
+    W = kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], theta, radius)
+    assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
+    assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
+    assert np.allclose(np.asarray(W.sum(axis=1)).ravel()[np.diff(rp) > 0], 1.0, atol=1e-12)

@@ -65,6 +65,7 @@ def test_dualgpm_cli_dual_resolution(tmp_path, gpu_available, tool):
             "--output", base]
     r = _run(args)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "W layout: stencil for" in r.stdout
     for suffix in ("_posterior_mean", "_posterior_variance", "_posterior_activation", "_residual"):
         assert os.path.exists(base + suffix + ".nii.gz"), suffix
     mean, q = read_nifti(base + "_posterior_mean.nii.gz")
@@ -202,3 +203,94 @@ def test_dualgpm_cli_update_tau(tmp_path, gpu_available):
     np.testing.assert_allclose(taus[False], np.sqrt(THETA[0]), rtol=1e-6)
     assert np.all(taus[True] > 0) and np.all(np.isfinite(taus[True]))
     assert np.ptp(taus[True]) > 0
+
+
+def _oblique_inputs(tmp_path, dims, vox, quat, qfac=1.0, seed=21):
+    """Highres and 2x2x2-block stdres NIfTI images sharing one rotated,
+    anisotropic quaternion qform (NIfTI-1 method 2); the stdres voxel centres
+    sit half a highres voxel along each rotated axis from the highres origin."""
+    from tests.nifti_io import quaternion_matrix
+    p = problem.dual_problem(dims, vox, THETA, seed_gp=seed, seed_noise=(seed + 1, seed + 2))
+    R = quaternion_matrix(*quat, vox, qfac)
+    o_hr = np.array([-91.3, 112.7, -54.1])
+    o_sr = o_hr + R @ np.array([0.5, 0.5, 0.5])
+    pix_s = tuple(2 * v for v in vox)
+    hr, sr = tmp_path / "hr.nii.gz", tmp_path / "sr.nii.gz"
+    write_nifti(str(hr), np.transpose(p["hr"], (2, 1, 0)), pixdim=vox, qoffset=tuple(o_hr), quatern=quat, qfac=qfac)
+    write_nifti(str(sr), np.transpose(p["std"], (2, 1, 0)), pixdim=pix_s, qoffset=tuple(o_sr), quatern=quat,
+                qfac=qfac)
+    _, q_hr = read_nifti(str(hr))
+    _, q_sr = read_nifti(str(sr))
+    return p, hr, sr, q_hr, q_sr
+
+
+@pytest.mark.gpu
+def test_dualgpmf_patient_geometry(tmp_path, gpu_available):
+    """An end-to-end dual-resolution dualgpmf fit on the patient01 header
+    geometry: 120x120x72 voxels of 1.8x1.8x2.3 mm under a quaternion qform
+    with a rotation (the non-axis-aligned branches of the torus base,
+    circulant_base.inl:60-85, and of the kriging offsets), stdres at twice the
+    voxel size.  Outputs keep the input's oblique qform; the posterior mean is
+    finite on the mask and zero off it, the residual is data minus mean, and
+    the fit lies closer to the underlying GP draw than the data do."""
+    quat = (0.061, -0.034, 0.118)
+    p, hr, sr, q_hr, q_sr = _oblique_inputs(tmp_path, (120, 120, 72), (1.8, 1.8, 2.3), quat)
+    base = str(tmp_path / "pat")
+    r = _run([os.path.join(BIN, "dualgpmf"), "--highres", str(hr), "--stdres", str(sr), "--covariance",
+              *map(str, THETA), "--burnin", "20", "--nsave", "20", "--thin", "1", "--leapfrog", "10", "--seed", "5",
+              "--output", base], timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Neighborhood: 6.9" in r.stdout
+    mean, q = read_nifti(base + "_posterior_mean.nii.gz")
+    res, _ = read_nifti(base + "_residual.nii.gz")
+    np.testing.assert_allclose(q, q_hr, atol=1e-5)
+    assert abs(q[0, 1]) > 0.1 and abs(q[1, 0]) > 0.1  # rotated, not axis-aligned
+    mask = np.transpose(p["mask"], (2, 1, 0))
+    y = np.transpose(p["hr"], (2, 1, 0))
+    truth = np.transpose(p["truth"], (2, 1, 0))
+    assert np.all(np.isfinite(mean[mask])) and np.all(mean[~mask] == 0)
+    np.testing.assert_allclose(res[mask], (y - mean)[mask], atol=1e-5 * (1 + np.abs(y[mask]).max()))
+    err_fit = np.sqrt(np.mean((mean[mask] - truth[mask]) ** 2))
+    err_data = np.sqrt(np.mean((y[mask] - truth[mask]) ** 2))
+    assert err_fit < err_data
+
+
+@pytest.mark.gpu
+def test_oblique_anisotropic_spectrum_kriging_and_gradient(tmp_path, gpu_available):
+    """The same rotated anisotropic geometry at 20x18x12 read back through the
+    NIfTI header: eigenvalues of the torus base Q (d(i), d(j), d(k)) against
+    the oracle (1e-12 of max|lambda|), the device kriging pattern bit-exact
+    against the oracle's integer search, and the float64 dual-resolution
+    gradient within 1e-10.  The k' distances here are float32 differences of
+    voxel positions about 100 mm from the origin (ulp 7.6e-6 mm), so the
+    weights follow the float32 rounding of the inputs: moving the stdres
+    origin by one ulp moves them by about 3e-6.  They are held to 1e-4 of the
+    largest weight, the level of that rounding (the axis-aligned test keeps
+    1e-6)."""
+    from helpers import oracle_chain, random_state, rel
+    from oracle import kriging as okr
+    quat = (0.061, -0.034, 0.118)
+    p, hr, sr, q_hr, q_sr = _oblique_inputs(tmp_path, (20, 18, 12), (1.8, 1.8, 2.3), quat, qfac=-1.0)
+    q_hr = q_hr.astype(np.float32)
+    q_sr = q_sr.astype(np.float32)
+    rp, cols, vals, nrow, ncol, st = okr.get_sparse_kriging_matrix_data(p["hr"], p["std"], q_hr, q_sr, THETA, 6.9)
+    W = kriging.kriging_build(p["hr"], q_hr, p["std"], q_sr, THETA, 6.9)
+    assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
+    assert np.max(np.abs(W.data - vals)) <= 1e-4 * np.abs(vals).max()
+    q = dict(p, q_hr=q_hr, W=W)
+    o, data = oracle_chain(q, THETA)
+    from dualres_amd import Chain
+    c = Chain(q["ijk"], THETA, q_hr, n_datasets=2, precision="fp64", state_mode="compat", seed=2)
+    c.set_data(q["Yh"], q["Ys"], W)
+    lam = o.lam.real
+    assert np.max(np.abs(c.lambda_() - lam)) <= 1e-12 * np.abs(lam).max()
+    assert c.n_nonpositive == o.Z.size
+    rng = np.random.default_rng(3)
+    mu, mom = random_state(o, rng)
+    mu.reshape(-1)[o.I] += q["Yh"]
+    o.mu = mu.copy()
+    o.sigma_sq_inv = [np.float64(1.2), np.float64(0.8)]
+    c.set_state(mu, mom)
+    c.set_sigma_sq_inv([1.2, 0.8])
+    assert rel(c.gradient().real, o.compute_gradient(data, o.mu).real) < 1e-10
+    c.close()

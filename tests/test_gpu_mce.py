@@ -54,3 +54,37 @@ def test_mce_binned_pools_offsets(gpu_available):
         if nn > 1:
             ref = (s[sel, 0].sum() - s[sel, 1].sum() * s[sel, 2].sum() / nn) / (nn - 1)
             assert abs(cov[k] - ref) <= 1e-9 * abs(cov[0]) + 1e-12
+
+
+def test_mce_full_size_closed_forms(gpu_available):
+    """At the estimation batch's full image size, 128x160x128: with every
+    voxel present the pair count of offset p is prod_a (n_a - |p_a|) for all
+    25,348 offsets (exact), a constant image has covariance 0 at every offset,
+    and the 64 bins to 20 mm pool exactly the closed-form counts of their
+    half-space offsets."""
+    dims = (128, 160, 128)
+    q = problem.qform_axis(1.0, dims)
+    rng = np.random.default_rng(2)
+    img = rng.standard_normal(dims[::-1]) + 5.0
+    P = mce_offsets().astype(np.int64)
+    want = np.prod(np.array(dims)[None, :] - np.abs(P), axis=1)
+    cov, dist, n = mce_summary(img, q)
+    assert np.array_equal(n.astype(np.int64), want)
+    assert np.array_equal(dist, mce.offset_distances(P, q))
+    const = np.full(dims[::-1], 2.5)
+    cov_c, _, n_c = mce_summary(const, q)
+    assert np.array_equal(n_c.astype(np.int64), want)
+    assert np.max(np.abs(cov_c)) < 1e-9 * 2.5 ** 2
+    nb, md = 64, 20.0
+    bc, bd, bn = mce_binned(const, q, nbins=nb, maxdist=md)
+    r = 20
+    H = np.array([[x, y, z] for z in range(0, r + 1) for y in range(-r, r + 1) for x in range(-r, r + 1)
+                  if not (z == 0 and (y < 0 or (y == 0 and x < 0)))])
+    d = mce.offset_distances(H, q)
+    keep = d <= np.float32(md)
+    H, d = H[keep], d[keep]
+    b = np.minimum((d / (md / nb)).astype(int), nb - 1)
+    cnt = np.prod(np.array(dims)[None, :] - np.abs(H), axis=1)
+    for k in range(nb):
+        assert bn[k] == int(cnt[b == k].sum()), k
+    assert np.max(np.abs(bc[bn > 1])) < 1e-9 * 2.5 ** 2

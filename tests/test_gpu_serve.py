@@ -68,3 +68,24 @@ def test_request_over_budget_is_refused(gpu_available):
     with serve.Server(1024, workers=1) as srv:
         out = srv.wait(srv.submit(r), timeout=30)
     assert out["status"] == 5 and "budget" in out["error"]
+
+
+@pytest.mark.gpu
+def test_device_budget_alone_limits_concurrency(gpu_available):
+    """workers <= 0: the server starts a thread for every request its device
+    budget admits.  With room for two of three equal fits, two run side by
+    side, the third waits for one of them, and all three give the same
+    result."""
+    reqs = [_requests()[1]] * 3
+    need = serve.estimate_bytes(reqs[0])
+    with serve.Server(int(2.5 * need), workers=0, backfill=False) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        res = [srv.wait(i, timeout=300) for i in ids]
+    assert all(r["status"] == 0 for r in res)
+    assert all(np.array_equal(r["mean"], res[0]["mean"]) for r in res)
+    starts = np.array([r["queue_s"] for r in res])
+    ends = starts + np.array([r["run_s"] for r in res])
+    order = np.argsort(starts)
+    first, second, third = order
+    assert starts[second] < ends[first] - 0.05            # the first two overlap
+    assert starts[third] >= min(ends[first], ends[second]) - 0.05
