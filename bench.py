@@ -62,36 +62,17 @@ def make_chain(p, precision, mode, seed=8675309):
 
 
 def run_chain(c, p, steps, warmup, seed=8675309):
-    """warmup iterations of dual averaging, then `steps` timed post-burn-in iterations."""
-    from dualres_amd.sampler import HMCParameters, McmcSummary, gibbs_sigma, init_sigma
-    rng = np.random.default_rng(seed)
-    init_sigma(c, rng, THETA[0])
-    hmc = HMCParameters(burnin=warmup, n_save=max(1, -(-steps // THIN)), thin=THIN, integrator_steps=L_STEPS,
-                        starting_eps=1.0 / L_STEPS, mh_target=0.65)
-    summ = McmcSummary(c.n_out, 2, chain=c)
-    n_s = p["Ys"].size
-    lat = []
-    grads = 0
-    t0 = None
-    while hmc.iteration < warmup + steps:
-        timed = hmc.iteration >= warmup
-        if timed and t0 is None:
-            t0 = time.perf_counter()
-        ti = time.perf_counter()
-        eps = hmc.eps(rng.uniform(0.9, 1.1))
-        L = hmc.integrator_steps()
-        r = c.update(eps, L, rng.uniform())
-        hmc.update(r["mh_rate"])
-        s = gibbs_sigma(c, rng, c.n_h, n_s, r["sse_h"], r["sse_s"])
-        if hmc.save_iteration():
-            summ.update(None, np.sqrt(1.0 / np.asarray(s)), -0.5 * (s[0] * r["sse_h"] + s[1] * r["sse_s"]))
-        if timed:
-            grads += L + 1
-            lat.append(time.perf_counter() - ti)
-    wall = time.perf_counter() - t0
-    return {"grad_evals_s": grads / wall, "wall": wall, "grads": grads, "ms_per_iter": 1e3 * wall / steps,
-            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-            "mh_rate": hmc.metropolis_hastings_rate(), "eps": hmc.eps_value(), "mean": summ.mode_mu()}
+    """warmup burn-in iterations of dual averaging, then `steps` timed
+    post-burn-in iterations, through the C++ host's fit loop (dualgpmf's
+    fit_model iterations: eps jitter, HMC update, Gibbs draws, saves every
+    THIN-th iteration)."""
+    from dualres_amd.sampler import fit_native
+    r = fit_native(c, p["Ys"].size, THETA[0], burnin=warmup, nsave=max(1, -(-steps // THIN)), thin=THIN, L=L_STEPS,
+                   seed=seed, post_iters=steps)
+    wall = r["wall_s"]
+    return {"grad_evals_s": r["grad_evals"] / wall, "wall": wall, "grads": r["grad_evals"],
+            "ms_per_iter": 1e3 * wall / r["iterations"], "p50_ms": 1e3 * r["p50_s"], "p99_ms": 1e3 * r["p99_s"],
+            "mh_rate": r["mh_rate"], "eps": r["eps"]}
 
 
 def kernel_bytes(name, p, prec_bytes, c):
@@ -304,7 +285,35 @@ def batch_problems(n_chains=64, seed=7):
     return out
 
 
-def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, seed=900, tune=True):
+def tuner_study(probs, iters=6):
+    """Per chain, the device time of `iters` HMC iterations (L = 25) with the
+    built-in FFT pass shapes and with the autotuner's, on its own stream."""
+    from dualres_amd import Chain
+    out = []
+    for q in probs:
+        c = Chain(q["ijk"], q["theta"], q["q_hr"], n_datasets=2, precision="fp32", state_mode="compat",
+                  embed=q["embed"], seed=11)
+        c.set_data(q["Yh"], q["Ys"], q["W"])
+        c.add_data_to_mu()
+        c.set_sigma_sq_inv([1.0, 1.0])
+        ts = []
+        for tuned in (False, True):
+            if tuned:
+                prm, _ = c.autotune()
+            c.update(0.04, L_STEPS, 0.0)
+            t = time.perf_counter()
+            for _ in range(iters):
+                c.update(0.04, L_STEPS, 0.0)
+            ts.append((time.perf_counter() - t) / iters)
+        out.append({"embed": list(q["embed"]), "default_ms": round(1e3 * ts[0], 3), "tuned_ms": round(1e3 * ts[1], 3),
+                    "params": prm})
+        c.close()
+    sp = [o["default_ms"] / o["tuned_ms"] for o in out]
+    log(f"[bench] tuner: single-chain speed-up median {np.median(sp):.3f} (min {min(sp):.3f}, max {max(sp):.3f})")
+    return {"chains": out, "speedup_median": round(float(np.median(sp)), 3)}
+
+
+def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, seed=900, tune=False):
     """Post-burn-in grad evals/s of all chains fitted in lockstep through one
     Batch (fit_batch: per chain dual averaging, Gibbs draws and saves; FP32
     gradients, FP64 energies), L = 25, in each batch mode; a chain's posterior
@@ -354,6 +363,7 @@ def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, s
     best = max(out["modes"], key=lambda m: out["modes"][m]["post_burnin_grad_evals_per_s"])
     out["best_mode"] = best
     out["post_burnin_grad_evals_per_s"] = out["modes"][best]["post_burnin_grad_evals_per_s"]
+    out["tuner"] = tuner_study(probs[:8])
     return out
 
 
@@ -527,7 +537,7 @@ def main():
     ap.add_argument("--mce-only", action="store_true", help="run only the parameter-estimation leg and print its JSON")
     ap.add_argument("--mce-images", type=int, default=32, help="images in the parameter-estimation leg")
     ap.add_argument("--batch-modes", default="graph,streams,eager", help="batch modes to time, in order")
-    ap.add_argument("--no-tune", action="store_true", help="batched chains keep the built-in FFT pass shapes")
+    ap.add_argument("--tune", action="store_true", help="autotune the batched chains' FFT pass shapes before fitting")
     ap.add_argument("--cpu-study", action="store_true",
                     help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
                          "medians of 3 iterations) and print its JSON")
@@ -551,7 +561,7 @@ def main():
         return
     if a.batch_only:
         print(json.dumps(batch_leg(batch_problems(), modes=tuple(a.batch_modes.split(",")),
-                                   tune=not a.no_tune)), flush=True)
+                                   tune=a.tune)), flush=True)
         return
     results = {}
     main_chain = None
@@ -590,7 +600,7 @@ def main():
         rates=tuple(float(x) for x in a.serve_rates.split(",")), n_req=a.serve_requests, nsave=a.serve_nsave,
         burnin=a.serve_nsave)
     batch = None if a.main_only or a.no_serve else batch_leg(batch_problems(), modes=tuple(a.batch_modes.split(",")),
-                                                             tune=not a.no_tune)
+                                                             tune=a.tune)
     estimation = None if a.main_only or a.no_serve else mce_leg(a.mce_images)
     cpu = None
     if not a.no_cpu:

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdualres_hip.so")
 
-DR_OK, DR_EDOMAIN, DR_ELOGIC, DR_ERUNTIME, DR_EHIP, DR_ENOMEM, DR_EBUSY = range(7)
+DR_OK, DR_EDOMAIN, DR_ELOGIC, DR_ERUNTIME, DR_EHIP, DR_ENOMEM, DR_EBUSY, DR_ECANCELED = range(8)
 DR_FP64, DR_FP32 = 0, 1
 DR_STATE_COMPAT, DR_STATE_REAL = 0, 1
 DR_COV_RBF, DR_COV_RQ = 0, 1

@@ -129,6 +129,25 @@ __host__ __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t str
   z1 = rad * s;
 }
 
+// ---------------------------------------------------------------- separately rounded float ops
+// HIP's __fmul_rn / __fadd_rn are plain '*' and '+', which hipcc's default
+// -ffp-contract=fast fuses into v_fma_f32 once inlined.  The reference's
+// distances are separately rounded float products and sums (x86-64 without
+// FMA), so these carry the contraction switch off in their own bodies: the
+// IR operations lose the contract flag and no later fusion can merge them.
+__host__ __device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__host__ __device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__host__ __device__ __forceinline__ float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+
 // ---------------------------------------------------------------- exact float sqrt
 // Correctly rounded float square root.  The hardware estimate is within a
 // couple of ulp; it is corrected with the exact midpoint tests

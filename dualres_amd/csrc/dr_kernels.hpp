@@ -358,12 +358,12 @@ struct LoadBase : NoRed {
     float c[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      float v = __fmul_rn(q[3 * r + 0], dx);
-      v = __fadd_rn(v, __fmul_rn(q[3 * r + 1], dy));
-      v = __fadd_rn(v, __fmul_rn(q[3 * r + 2], dz));
+      float v = mul_rn(q[3 * r + 0], dx);
+      v = add_rn(v, mul_rn(q[3 * r + 1], dy));
+      v = add_rn(v, mul_rn(q[3 * r + 2], dz));
       c[r] = v;
     }
-    const float ss = __fadd_rn(__fadd_rn(__fmul_rn(c[0], c[0]), __fmul_rn(c[1], c[1])), __fmul_rn(c[2], c[2]));
+    const float ss = add_rn(add_rn(mul_rn(c[0], c[0]), mul_rn(c[1], c[1])), mul_rn(c[2], c[2]));
     const double d = (double)sqrt_rn(ss);
     if (covfun == 0) return tau2 * exp(-psi * pow(fabs(d), nu));
     return tau2 * pow(1.0 + psi / nu * d * d, -nu);

@@ -8,7 +8,8 @@
 //     is then verified element by element on the device (k_verify).
 //  3. Unique systems, sorted by size and padded with an identity block to a
 //     multiple of 64, are factorised in batches by a blocked right-looking
-//     float64 Cholesky (k_potrf / k_trsm / k_syrk over 64x64 tiles) and
+//     float64 Cholesky (k_potrf / k_trsm over 64x64 tiles, the trailing
+//     update k_syrk_mfma on the FP64 matrix cores) and
 //     solved (k_trisolve), then normalised by their sum (kriging_matrix.h:318-321).
 //  4. Rows gather their system's weights; rows whose weights sum to NaN stay
 //     empty, as in the reference.
@@ -45,7 +46,7 @@ struct Geo {
 __device__ __forceinline__ float dot4(const float* q, int r, const float* v) {
   float acc = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) acc = __fadd_rn(acc, __fmul_rn(q[4 * r + c], v[c]));
+  for (int c = 0; c < 4; ++c) acc = add_rn(acc, mul_rn(q[4 * r + c], v[c]));
   return acc;
 }
 // host twin (x86-64 without FMA: separately rounded products and sums)
@@ -119,10 +120,10 @@ k_pattern(Geo g, int n_rows, const int* __restrict__ sijk, const int* __restrict
       float ss = 0.0f;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float d = __fsub_rn(xs[r], dot4(g.qh, r, h4));
-        ss = __fadd_rn(ss, __fmul_rn(d, d));
+        const float d = sub_rn(xs[r], dot4(g.qh, r, h4));
+        ss = add_rn(ss, mul_rn(d, d));
       }
-      k = kern_f(g, __fadd_rn(sqrt_rn(ss), g.eps0));
+      k = kern_f(g, add_rn(sqrt_rn(ss), g.eps0));
     }
     const uint64_t bal = __ballot(valid);
     const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
@@ -262,8 +263,16 @@ __global__ void __launch_bounds__(256) k_trsm(double* A, int np, int k) {
   }
 }
 
-// trailing update of lower tiles: A[BI][BJ] -= P[BI] P[BJ]^T
-__global__ void __launch_bounds__(256) k_syrk(double* A, int np, int k) {
+// Trailing update of the lower tiles, A[BI][BJ] -= P[BI] P[BJ]^T, on the FP64
+// matrix cores: each wavefront owns a 32x32 quadrant of the 64x64 tile as 2x2
+// blocks of 16x16, accumulated by v_mfma_f64_16x16x4f64 over the panel's 64
+// columns, 4 at a time.  Operand layout of the instruction (checked on the
+// device with distinct per-lane values): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator register j of lane l holds
+// D[4 j + l / 16][l % 16].  Here A = P[BI] (rows of the quadrant, panel
+// columns as k) and B = P[BJ]^T.
+typedef double dr_v4f64 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_syrk_mfma(double* A, int np, int k) {
   __shared__ double Pi[NB][NB + 1];
   __shared__ double Pj[NB][NB + 1];
   double* As = A + (long)blockIdx.x * np * np;
@@ -279,27 +288,35 @@ __global__ void __launch_bounds__(256) k_syrk(double* A, int np, int k) {
     Pj[i][l] = As[(long)(BJ + i) * np + o + l];
   }
   __syncthreads();
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  dr_v4f64 acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-  for (int l = 0; l < NB; ++l) {
-    double a[4], b[4];
+    for (int b = 0; b < 2; ++b) acc[a][b] = dr_v4f64{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < NB; kk += 4) {
+    double av[2], bv[2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[r] = Pi[ty * 4 + r][l];
+    for (int a = 0; a < 2; ++a) av[a] = Pi[r0 + 16 * a + li][kk + lk];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) b[c] = Pj[tx * 4 + c][l];
+    for (int b = 0; b < 2; ++b) bv[b] = Pj[c0 + 16 * b + li][kk + lk];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) As[(long)(BI + ty * 4 + r) * np + BJ + tx * 4 + c] -= acc[r][c];
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = r0 + 16 * a + 4 * j + lk, col = c0 + 16 * b + li;
+        As[(long)(BI + row) * np + BJ + col] -= acc[a][b][j];
+      }
 }
 
 __device__ __forceinline__ double wsum(double v) {
@@ -671,7 +688,7 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
       const int m = nt - k - 1;
       if (m > 0) {
         hipLaunchKernelGGL(k_trsm, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
-        hipLaunchKernelGGL(k_syrk, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
+        hipLaunchKernelGGL(k_syrk_mfma, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
       }
       DR_HIP(hipGetLastError());
     }

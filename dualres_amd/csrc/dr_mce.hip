@@ -346,8 +346,8 @@ __global__ void k_mce_finalize(int n, const double* __restrict__ sums, const int
   float ss = 0.0f;
   for (int r = 0; r < 4; ++r) {
     float v = 0.0f;
-    for (int c = 0; c < 3; ++c) v = __fadd_rn(v, __fmul_rn(qf.q[4 * r + c], (float)off[3 * i + c]));
-    ss = __fadd_rn(ss, __fmul_rn(v, v));
+    for (int c = 0; c < 3; ++c) v = add_rn(v, mul_rn(qf.q[4 * r + c], (float)off[3 * i + c]));
+    ss = add_rn(ss, mul_rn(v, v));
   }
   dist[i] = sqrt_rn(ss);
 }

@@ -62,8 +62,11 @@ class Chain {
     check(dr_create(&cfg, flat.data(), n_h_, &h_));
     nd_ = cfg.n_datasets;
   }
+  // a handle created elsewhere (the caller keeps ownership)
+  Chain(dr_chain* borrowed, int n_datasets, long n_h, long n_out)
+      : h_(borrowed), n_h_(n_h), n_out_(n_out), nd_(n_datasets), owned_(false) {}
   ~Chain() {
-    if (h_) dr_destroy(h_);
+    if (h_ && owned_) dr_destroy(h_);
   }
   Chain(const Chain&) = delete;
   Chain& operator=(const Chain&) = delete;
@@ -117,6 +120,7 @@ class Chain {
   dr_chain* h_ = nullptr;
   long n_h_ = 0, n_out_ = -1;
   int nd_ = 1;
+  bool owned_ = true;
 };
 
 }  // namespace drhost

@@ -1,10 +1,13 @@
 // C entry points of the host layer, for tests and embedding
 // (libdualres_host.so): the RBF minimum-contrast fit and NIfTI I/O.
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "../../include/dualres_host.h"
 #include "nifti.hpp"
 #include "rbf_fit.hpp"
 #include "sampler.hpp"
@@ -60,6 +63,73 @@ int drh_hmc_trace(int burnin, int n_save, int thin, int integrator_steps, double
     return 0;
   } catch (const std::domain_error&) {
     return 2;
+  }
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+void fit_chain(dr_chain* c, int nd, int64_t n_h, int64_t n_s, int64_t n_out, double tau2, int burnin, int nsave,
+               int thin, int L, uint64_t seed, int64_t post_iters, double* out) {
+  using namespace drhost;
+  rng().seed((std::mt19937::result_type)seed);
+  Chain chain(c, nd, (long)n_h, (long)n_out);
+  HMCParameters<T> hmc(burnin, nsave, thin, L, (T)(1.0 / L), (T)0.65);
+  // _initialize_sigma (MultiResParameters.h:764-778), as fit_model draws it
+  std::vector<double> sig(nd);
+  std::gamma_distribution<T> G((T)tau2, (T)1);
+  std::uniform_real_distribution<T> U(0, 1);
+  sig[0] = G(rng());
+  for (int i = 1; i < nd; ++i) sig[i] = (T)sig[0] / U(rng());
+  chain.set_sigma_sq_inv(sig);
+  FitOptions opt;
+  opt.quiet = true;
+  NullStream quiet;
+  std::ofstream none;
+  std::vector<double> lat;
+  McmcSummary<T> summ = sample_chain<T>(chain, (long)n_h, (long)n_s, hmc, sig, none, opt, quiet, &lat, (long)post_iters);
+  double wall = 0;
+  for (double t : lat) wall += t;
+  std::vector<double> sorted = lat;
+  std::sort(sorted.begin(), sorted.end());
+  auto pct = [&](double q) {
+    if (sorted.empty()) return 0.0;
+    const double pos = q * (sorted.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - i;
+    return i + 1 < sorted.size() ? sorted[i] * (1 - f) + sorted[i + 1] * f : sorted[i];
+  };
+  out[0] = wall;
+  out[1] = (double)lat.size() * (L + 1);
+  out[2] = hmc.metropolis_hastings_rate();
+  out[3] = hmc.eps_value();
+  out[4] = pct(0.5);
+  out[5] = pct(0.99);
+  out[6] = (double)lat.size();
+}
+}  // namespace
+
+extern "C" {
+
+// The fit loop of dualgpm / dualgpmf (fit_model's HMC iterations, Gibbs draws,
+// dual averaging and device summaries, gaussian_process_model.h:134-171) on a
+// chain created elsewhere with its data set, mu holding the data and its
+// output indices set.  precision 0 runs the host scalars in double as
+// dualgpm, 1 in float as dualgpmf.  post_iters > 0 stops after that many
+// post-burn-in iterations.  out[7]: post-burn-in wall seconds, gradient
+// evaluations, MH rate, final eps, p50 and p99 iteration seconds, iterations.
+int drh_fit_chain(dr_chain* c, int precision, int n_datasets, int64_t n_h, int64_t n_s, int64_t n_out, double tau2,
+                  int burnin, int nsave, int thin, int L, uint64_t seed, int64_t post_iters, double* out) {
+  try {
+    if (precision == 0)
+      fit_chain<double>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, out);
+    else
+      fit_chain<float>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, out);
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "drh_fit_chain: %s\n", e.what());
+    return 1;
   }
 }
 

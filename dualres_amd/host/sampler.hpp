@@ -7,6 +7,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -69,6 +70,7 @@ class HMCParameters {
   int iteration() const { return iteration_; }
   int max_iterations() const { return burnin_ + thin_ * n_save_; }
   int n_save() const { return n_save_; }
+  int burnin() const { return burnin_; }
   value_type eps() const {
     std::uniform_real_distribution<> U(0.9, 1.1);
     if (warmup_) return (value_type)eps_;
@@ -173,7 +175,18 @@ struct FitOptions {
   // Gibbs update of tau^-2 after the sigma draws (MultiResParameters.h:723-725),
   // which both reference binaries compile out (DUALRES_DONT_UPDATE_TAU)
   bool update_tau = false;
+  // set by another thread to end the fit: checked before the kriging build
+  // and before every HMC iteration, which then throws Cancelled
+  const std::atomic<bool>* cancel = nullptr;
 };
+
+struct Cancelled : std::runtime_error {
+  Cancelled() : std::runtime_error("fit cancelled") {}
+};
+
+inline void check_cancel(const FitOptions& opt) {
+  if (opt.cancel && opt.cancel->load(std::memory_order_relaxed)) throw Cancelled();
+}
 
 struct MultiResData {  // MultiResData.h:21-127
   int n_datasets = 1;
@@ -181,6 +194,83 @@ struct MultiResData {  // MultiResData.h:21-127
   Csr W;
   std::vector<double> theta;
 };
+
+// The post-setup loop of fit_model (gaussian_process_model.h:134-171) on a
+// prepared chain (data, mu with data added, output indices, initial sigma
+// draws in sig): HMC update, Gibbs draws (and tau with opt.update_tau), dual
+// averaging and saves.  iter_s, when given, receives the wall seconds of
+// every post-burn-in iteration; stop_after > 0 ends the loop after that many
+// post-burn-in iterations.
+template <typename T>
+McmcSummary<T> sample_chain(Chain& chain, long n_h, long n_s, HMCParameters<T>& hmc, std::vector<double> sig,
+                            std::ofstream& samples_out, const FitOptions& opt, std::ostream& out,
+                            std::vector<double>* iter_s, long stop_after = -1) {
+  const int nd = chain.n_datasets();
+  McmcSummary<T> summ(chain.n_out(), nd);
+  int save_count = 0;
+  out << "\nFitting model with rmHMC";
+  if (nd > 1) out << " and projected neighborhood process approximation";
+  out << std::endl;
+  auto start = std::chrono::high_resolution_clock::now();
+  std::uniform_real_distribution<double> U01(0, 1);
+  std::vector<T> sigma(nd);
+  while (save_count < hmc.n_save() && hmc.iteration() <= hmc.max_iterations()) {
+    check_cancel(opt);
+    const bool post = hmc.iteration() >= hmc.burnin();
+    const auto t_iter = std::chrono::steady_clock::now();
+    const T eps = hmc.eps();
+    const int L = hmc.integrator_steps();
+    const double u = U01(rng());
+    const dr_update_result r = chain.update(eps, L, u);
+    if (opt.monitor)
+      out << std::setprecision(4) << "Energy Transition (" << r.H0 << " -> " << r.H1 << ");  step size ("
+                << eps << ");  MH = (" << r.R << ")" << std::endl;
+    // _update_sigma_sq_inv (MultiResParameters.h:820-859)
+    {
+      std::gamma_distribution<T> G((T)(0.5 * n_h + 1), (T)(1 / (T)(0.5 * r.sse_h)));
+      sig[0] = G(rng());
+      if (nd == 2) {
+        std::gamma_distribution<T> G2((T)(0.5 * n_s + 1), (T)(1 / (T)(0.5 * r.sse_s)));
+        sig[1] = G2(rng());
+      }
+      chain.set_sigma_sq_inv(sig);
+    }
+    // _update_tau_sq_inv (MultiResParameters.h:863-872)
+    if (opt.update_tau) {
+      const T shape = (T)(0.5 * n_h + 1);
+      const T rate = (T)(0.5 * chain.mu_data_sumsq());
+      std::gamma_distribution<T> G(shape, (T)(1 / rate));
+      chain.set_tau_sq_inv(G(rng()));
+    }
+    hmc.update((T)r.mh_rate);
+    if (hmc.save_iteration()) {
+      if (save_count == 0) start = std::chrono::high_resolution_clock::now();
+      for (int i = 0; i < nd; ++i) sigma[i] = (T)std::sqrt(1.0 / sig[i]);
+      double ll = sig[0] * r.sse_h;
+      if (nd == 2) ll += sig[1] * r.sse_s;
+      const T log_lik = (T)(-0.5 * ll);
+      summ.update(chain, sigma, log_lik);
+      if (samples_out.is_open() && opt.output_samples) {
+        const std::vector<double> mu = chain.mu_();
+        for (double v : mu) samples_out << (T)v << "\t";
+        for (int i = 0; i < nd; ++i) samples_out << sigma[i] << "\t";
+        samples_out << (T)std::sqrt(1.0 / chain.tau_sq_inv()) << "\t" << log_lik;
+        if (save_count < hmc.n_save() - 1) samples_out << std::endl;
+      }
+      ++save_count;
+    }
+    if (post && iter_s)
+      iter_s->push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count());
+    if (post && stop_after > 0 && --stop_after == 0) break;
+  }
+  const auto stop = std::chrono::high_resolution_clock::now();
+  summ.finish(chain);
+  summ.sampling_time = (T)(std::chrono::duration_cast<std::chrono::milliseconds>(stop - start).count() / 1e3);
+  T rate = hmc.metropolis_hastings_rate();
+  if (std::isnan(rate)) rate = 0;
+  if (rate >= 0 && rate <= 1) summ.mh_rate = rate;
+  return summ;
+}
 
 // fit_model (gaussian_process_model.h:87-177)
 template <typename T>
@@ -240,64 +330,8 @@ McmcSummary<T> fit_model(const Image& hr, const Image& input_mask, const Image& 
   chain.set_sigma_sq_inv(sig);
   chain.add_data_to_mu();
   chain.set_output_indices(nonzero_indices_bounded_by_box(output_mask, bb));
-  McmcSummary<T> summ(chain.n_out(), data.n_datasets);
-  int save_count = 0;
-  out << "\nFitting model with rmHMC";
-  if (data.n_datasets > 1) out << " and projected neighborhood process approximation";
-  out << std::endl;
-  auto start = std::chrono::high_resolution_clock::now();
-  std::uniform_real_distribution<double> U01(0, 1);
-  std::vector<T> sigma(data.n_datasets);
-  while (save_count < hmc.n_save() && hmc.iteration() <= hmc.max_iterations()) {
-    const T eps = hmc.eps();
-    const int L = hmc.integrator_steps();
-    const double u = U01(rng());
-    const dr_update_result r = chain.update(eps, L, u);
-    if (opt.monitor)
-      out << std::setprecision(4) << "Energy Transition (" << r.H0 << " -> " << r.H1 << ");  step size ("
-                << eps << ");  MH = (" << r.R << ")" << std::endl;
-    // _update_sigma_sq_inv (MultiResParameters.h:820-859)
-    {
-      std::gamma_distribution<T> G((T)(0.5 * data.Yh.size() + 1), (T)(1 / (T)(0.5 * r.sse_h)));
-      sig[0] = G(rng());
-      if (data.n_datasets == 2) {
-        std::gamma_distribution<T> G2((T)(0.5 * data.Ys.size() + 1), (T)(1 / (T)(0.5 * r.sse_s)));
-        sig[1] = G2(rng());
-      }
-      chain.set_sigma_sq_inv(sig);
-    }
-    // _update_tau_sq_inv (MultiResParameters.h:863-872)
-    if (opt.update_tau) {
-      const T shape = (T)(0.5 * data.Yh.size() + 1);
-      const T rate = (T)(0.5 * chain.mu_data_sumsq());
-      std::gamma_distribution<T> G(shape, (T)(1 / rate));
-      chain.set_tau_sq_inv(G(rng()));
-    }
-    hmc.update((T)r.mh_rate);
-    if (hmc.save_iteration()) {
-      if (save_count == 0) start = std::chrono::high_resolution_clock::now();
-      for (int i = 0; i < data.n_datasets; ++i) sigma[i] = (T)std::sqrt(1.0 / sig[i]);
-      double ll = sig[0] * r.sse_h;
-      if (data.n_datasets == 2) ll += sig[1] * r.sse_s;
-      const T log_lik = (T)(-0.5 * ll);
-      summ.update(chain, sigma, log_lik);
-      if (samples_out.is_open() && opt.output_samples) {
-        const std::vector<double> mu = chain.mu_();
-        for (double v : mu) samples_out << (T)v << "\t";
-        for (int i = 0; i < data.n_datasets; ++i) samples_out << sigma[i] << "\t";
-        samples_out << (T)std::sqrt(1.0 / chain.tau_sq_inv()) << "\t" << log_lik;
-        if (save_count < hmc.n_save() - 1) samples_out << std::endl;
-      }
-      ++save_count;
-    }
-  }
-  const auto stop = std::chrono::high_resolution_clock::now();
-  summ.finish(chain);
-  summ.sampling_time = (T)(std::chrono::duration_cast<std::chrono::milliseconds>(stop - start).count() / 1e3);
-  T rate = hmc.metropolis_hastings_rate();
-  if (std::isnan(rate)) rate = 0;
-  if (rate >= 0 && rate <= 1) summ.mh_rate = rate;
-  return summ;
+  return sample_chain<T>(chain, (long)data.Yh.size(), (long)data.Ys.size(), hmc, sig, samples_out, opt, out,
+                         nullptr);
 }
 
 }  // namespace drhost

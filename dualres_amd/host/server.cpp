@@ -8,6 +8,7 @@
 #include <fstream>
 #include <stdexcept>
 
+#include "../../include/dualres_host.h"
 #include "engine.hpp"
 #include "nifti.hpp"
 #include "sampler.hpp"
@@ -55,8 +56,11 @@ dr_config make_config(const ServeRequest& r, int device) {
 }
 
 template <typename T>
-ServeResult fit_as(const ServeRequest& r, int device) {
+ServeResult fit_as(const ServeRequest& r, int device, const std::atomic<bool>* cancel) {
   ServeResult out;
+  FitOptions fo;
+  fo.cancel = cancel;
+  check_cancel(fo);
   rng().seed((std::mt19937::result_type)r.seed);
   const Image hr = make_image(r.hr, r.hd, r.qh);
   MultiResData data;
@@ -69,7 +73,7 @@ ServeResult fit_as(const ServeRequest& r, int device) {
     data.Ys = nonzero_data(sr);
   }
   HMCParameters<T> hmc(r.burnin, r.nsave, r.thin, r.L, (T)(1.0 / r.L), (T)r.mh_target);
-  FitOptions fo;
+  check_cancel(fo);
   fo.precision = r.precision;
   fo.state_mode = r.state_mode;
   fo.covfun = DR_COV_RBF;
@@ -113,11 +117,14 @@ int64_t estimate_request_bytes(const ServeRequest& r) {
   return b;
 }
 
-ServeResult run_fit(const ServeRequest& r, int device) {
+ServeResult run_fit(const ServeRequest& r, int device, const std::atomic<bool>* cancel) {
   ServeResult out;
   try {
-    out = r.precision == DR_FP64 ? fit_as<double>(r, device) : fit_as<float>(r, device);
+    out = r.precision == DR_FP64 ? fit_as<double>(r, device, cancel) : fit_as<float>(r, device, cancel);
     out.status = DR_OK;
+  } catch (const Cancelled& e) {
+    out.status = DR_ECANCELED;
+    out.error = e.what();
   } catch (const std::domain_error& e) {
     out.status = DR_EDOMAIN;
     out.error = e.what();
@@ -164,7 +171,7 @@ int64_t Server::submit(ServeRequest r) {
     cv_done_.notify_all();
     return id;
   }
-  queue_.push_back(Job{id, std::move(r), bytes, now_s()});
+  queue_.push_back(Job{id, std::move(r), bytes, now_s(), std::make_shared<std::atomic<bool>>(false)});
   cv_work_.notify_all();
   return id;
 }
@@ -183,12 +190,34 @@ bool Server::take(Job* job) {
   queue_.erase(it);
   in_use_ += job->bytes;
   ++running_;
+  running_flags_[job->id] = job->cancel;
   return true;
+}
+
+int Server::cancel(int64_t id) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto q = queue_.begin(); q != queue_.end(); ++q) {
+    if (q->id != id) continue;
+    ServeResult res;
+    res.status = DR_ECANCELED;
+    res.error = "request cancelled before it started";
+    res.device_bytes = q->bytes;
+    res.queue_s = res.total_s = now_s() - q->t_submit;
+    queue_.erase(q);
+    done_[id] = std::move(res);
+    cv_done_.notify_all();
+    cv_work_.notify_all();  // a smaller request behind it may now be admitted
+    return 0;
+  }
+  const auto f = running_flags_.find(id);
+  if (f == running_flags_.end()) return 2;
+  f->second->store(true);
+  return 1;
 }
 
 void Server::run_job(Job job) {
   const double t0 = now_s();
-  ServeResult res = run_fit(job.req, device_);
+  ServeResult res = run_fit(job.req, device_, job.cancel.get());
   const double t1 = now_s();
   res.queue_s = t0 - job.t_submit;
   res.run_s = t1 - t0;
@@ -198,6 +227,7 @@ void Server::run_job(Job job) {
     std::lock_guard<std::mutex> g(mu_);
     in_use_ -= job.bytes;
     --running_;
+    running_flags_.erase(job.id);
     done_[job.id] = std::move(res);
   }
   cv_done_.notify_all();
@@ -273,29 +303,6 @@ int Server::running() {
 // ---------------------------------------------------------------- C entry points
 extern "C" {
 
-struct drh_request {
-  int n_datasets;
-  const double* hr;
-  int hr_dims[3];
-  float q_hr[16];
-  const double* sr;
-  int sr_dims[3];
-  float q_sr[16];
-  double theta[3];
-  double neighborhood;
-  int burnin, nsave, thin, L;
-  double mh_target;
-  uint64_t seed;
-  int precision, state_mode;
-  int embed[3];
-};
-
-struct drh_result {
-  int status;
-  double mh_rate, eps, queue_s, run_s, total_s;
-  int64_t grad_evals, device_bytes, n_out;
-};
-
 static drhost::ServeRequest to_request(const drh_request* q) {
   drhost::ServeRequest r;
   r.n_datasets = q->n_datasets;
@@ -366,6 +373,9 @@ int drh_server_wait(void* s, int64_t id, double timeout_s, drh_result* res, doub
   }
   return 0;
 }
+
+// 0: dropped from the queue; 1: running fit told to stop; 2: unknown or done
+int drh_server_cancel(void* s, int64_t id) { return static_cast<drhost::Server*>(s)->cancel(id); }
 
 int drh_server_destroy(void* s) {
   delete static_cast<drhost::Server*>(s);

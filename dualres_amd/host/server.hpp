@@ -62,6 +62,11 @@ class Server {
   int64_t submit(ServeRequest r);  // returns the request id
   // false on timeout; the result is removed from the server when returned
   bool wait(int64_t id, ServeResult* out, double timeout_s);
+  // 0: the request was still queued and is dropped (its result, status
+  // DR_ECANCELED, is ready); 1: its fit is running and is told to stop at its
+  // next HMC iteration (the result then has DR_ECANCELED unless the fit ended
+  // first); 2: no such request, or its result is already there
+  int cancel(int64_t id);
   int64_t bytes_in_use();
   int running();
 
@@ -71,6 +76,7 @@ class Server {
     ServeRequest req;
     int64_t bytes;
     double t_submit;
+    std::shared_ptr<std::atomic<bool>> cancel;
   };
   void worker();
   void dispatcher();
@@ -87,6 +93,7 @@ class Server {
   int running_ = 0;
   std::deque<Job> queue_;
   std::map<int64_t, ServeResult> done_;
+  std::map<int64_t, std::shared_ptr<std::atomic<bool>>> running_flags_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::vector<std::thread> threads_;
@@ -94,7 +101,8 @@ class Server {
 
 // device bytes a request will hold at its peak (chain + W + kriging scratch)
 int64_t estimate_request_bytes(const ServeRequest& r);
-// runs one fit on the calling thread (what a worker does)
-ServeResult run_fit(const ServeRequest& r, int device);
+// runs one fit on the calling thread (what a worker does); cancel, when
+// given, ends it early with status DR_ECANCELED once set
+ServeResult run_fit(const ServeRequest& r, int device, const std::atomic<bool>* cancel = nullptr);
 
 }  // namespace drhost

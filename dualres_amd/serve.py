@@ -52,6 +52,8 @@ def load_host():
     lib.drh_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(_Result),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64, C.c_char_p, C.c_int]
     lib.drh_server_destroy.argtypes = [C.c_void_p]
+    lib.drh_server_cancel.argtypes = [C.c_void_p, C.c_int64]
+    lib.drh_server_cancel.restype = C.c_int
     lib.drh_estimate_request_bytes.argtypes = [C.POINTER(_Request), C.POINTER(C.c_int64)]
     return lib
 
@@ -135,6 +137,12 @@ class Server:
         out["error"] = err.value.decode()
         out["mean"], out["var"] = (mean, var) if res.status == 0 else (None, None)
         return out
+
+    def cancel(self, rid):
+        """0: dropped while queued; 1: running fit told to stop at its next
+        HMC iteration; 2: unknown id or already finished.  A cancelled
+        request's ``wait`` returns status DR_ECANCELED (7)."""
+        return int(self._lib.drh_server_cancel(self._h, int(rid)))
 
     def close(self):
         if self._h:

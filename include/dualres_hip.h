@@ -26,7 +26,8 @@ enum dr_status {
   DR_ERUNTIME = 3, /* std::runtime_error                                     */
   DR_EHIP = 4,     /* HIP runtime failure                                    */
   DR_ENOMEM = 5,   /* device or host allocation failed                       */
-  DR_EBUSY = 6     /* serving: admission refused / no slot                   */
+  DR_EBUSY = 6,    /* serving: admission refused / no slot                   */
+  DR_ECANCELED = 7 /* serving: the request was cancelled                     */
 };
 
 enum dr_precision {

@@ -29,6 +29,17 @@ def test_library_exports_every_declared_symbol():
     assert set(L.SIGNATURES) | {"dr_last_error"} >= set(names)
 
 
+def test_host_library_exports_every_declared_symbol():
+    import os
+    root = os.path.dirname(os.path.dirname(__file__))
+    src = open(os.path.join(root, "include", "dualres_host.h")).read()
+    names = sorted(set(re.findall(r"\b(drh_[a-z0-9_]+)\s*\(", src)))
+    assert len(names) >= 10
+    lib = ctypes.CDLL(os.path.join(root, "dualres_amd", "libdualres_host.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
 def test_argument_validation_without_device():
     lib = L.load()
     cfg = L.dr_config()

@@ -131,6 +131,40 @@ def test_update_injected_noise_fp64(gpu_available, dual):
     assert rel(mu_e.imag, o.mu.imag) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_nan_energy_is_rejected(gpu_available, precision):
+    """A proposal whose energy is NaN has R = NaN, which the reference turns
+    into R = 0 (MultiResParameters.h:714): it is rejected even with a zero
+    accept uniform, mh_rate is 0, the state is untouched, and the next
+    iteration is the one a chain that never saw the NaN would run.  The NaN
+    enters through one injected momentum normal."""
+    p = small_dual()
+    o, data, c, mu, _ = _setup(p, THETA2, precision, "compat")
+    rng = np.random.default_rng(8)
+    zr, zi = rng.standard_normal(o.N).reshape(o.shape), rng.standard_normal(o.N).reshape(o.shape)
+    bad = zr.copy()
+    bad.reshape(-1)[5] = np.nan
+    o.rng = mrp.ScriptedRng(normals=[(bad.reshape(-1), zi.reshape(-1))], uniforms=[0.0], gammas=[1.0, 1.0])
+    with np.errstate(invalid="ignore"):
+        mh_o = o.update(data, 0.05, 4)
+    assert mh_o == 0.0 and not o.accepted
+    mu0, _ = c.get_state()
+    r = c.update(0.05, 4, 0.0, noise=(bad, zi))
+    assert np.isnan(r["H0"]) or np.isnan(r["H1"])
+    assert r["mh_rate"] == 0.0 and not r["accepted"]
+    mu1, _ = c.get_state()
+    assert np.array_equal(mu1, mu0)
+    # the next iteration equals that of a twin chain set to the same state
+    _, _, twin, _, _ = _setup(p, THETA2, precision, "compat")
+    twin.set_sigma_sq_inv(c.sigma_sq_inv())
+    a = c.update(0.05, 4, 0.5, noise=(zr, zi))
+    b = twin.update(0.05, 4, 0.5, noise=(zr, zi))
+    assert a["H0"] == b["H0"] and a["H1"] == b["H1"] and a["accepted"] == b["accepted"]
+    assert np.array_equal(c.get_state()[0], twin.get_state()[0])
+    c.close()
+    twin.close()
+
+
 @pytest.mark.parametrize("dual", [False, True])
 def test_gradient_leapfrog_fp32(gpu_available, dual):
     p = small_dual() if dual else small_single()
@@ -504,3 +538,25 @@ def test_kriging_indefinite_systems_tau2_above_one(gpu_available):
     assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
     assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
     assert np.allclose(np.asarray(W.sum(axis=1)).ravel()[np.diff(rp) > 0], 1.0, atol=1e-12)
+
+
+def test_native_fit_loop_on_a_prepared_chain(gpu_available):
+    """The C++ host's fit loop (dualgpmf's fit_model iterations) driven on a
+    chain prepared through the C ABI: the same seed reproduces the run bit
+    for bit, post_iters bounds the timed iterations, the step size tunes
+    during burn-in and the acceptance rate is a probability."""
+    from dualres_amd.sampler import fit_native
+    p = small_dual(dims=(16, 14, 12))
+    outs = []
+    for _ in range(2):
+        c = engine_chain(p, THETA2, precision="fp32", state_mode="compat", seed=6)
+        c.add_data_to_mu()
+        c.set_output_indices(p["ijk"])
+        r = fit_native(c, p["Ys"].size, THETA2[0], burnin=20, nsave=10, thin=3, L=6, seed=12, post_iters=25)
+        outs.append((r, c.mu(), c.summary_read()))
+        c.close()
+    (r1, m1, s1), (r2, m2, s2) = outs
+    assert r1["iterations"] == 25 and r1["grad_evals"] == 25 * 7
+    assert 0.0 < r1["mh_rate"] <= 1.0 and r1["eps"] != 1.0 / 6
+    assert r1["mh_rate"] == r2["mh_rate"] and r1["eps"] == r2["eps"]
+    assert np.array_equal(m1, m2) and np.array_equal(s1[0], s2[0]) and s1[3] == 8

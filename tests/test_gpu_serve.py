@@ -75,8 +75,9 @@ def test_device_budget_alone_limits_concurrency(gpu_available):
     """workers <= 0: the server starts a thread for every request its device
     budget admits.  With room for two of three equal fits, two run side by
     side, the third waits for one of them, and all three give the same
-    result."""
-    reqs = [_requests()[1]] * 3
+    result.  The fits are long enough (a few hundred iterations) for the
+    overlap to be seen on the submit clock."""
+    reqs = [dict(_requests()[1], burnin=300, nsave=200)] * 3
     need = serve.estimate_bytes(reqs[0])
     with serve.Server(int(2.5 * need), workers=0, backfill=False) as srv:
         ids = [srv.submit(r) for r in reqs]
@@ -87,5 +88,34 @@ def test_device_budget_alone_limits_concurrency(gpu_available):
     ends = starts + np.array([r["run_s"] for r in res])
     order = np.argsort(starts)
     first, second, third = order
+    assert ends[first] - starts[first] > 0.1
     assert starts[second] < ends[first] - 0.05            # the first two overlap
-    assert starts[third] >= min(ends[first], ends[second]) - 0.05
+    assert starts[third] >= min(ends[first], ends[second]) - 0.01
+
+
+@pytest.mark.gpu
+def test_cancel_queued_and_running(gpu_available):
+    """A queued request is dropped at once (cancel returns 0); a running fit
+    stops at its next HMC iteration (cancel returns 1); both report
+    DR_ECANCELED, and the server goes on serving.  Unknown or finished ids
+    return 2."""
+    import time
+    base = _requests()[0]
+    long_fit = dict(base, burnin=200000, nsave=15)
+    with serve.Server(4 * serve.estimate_bytes(base), workers=1) as srv:
+        a = srv.submit(long_fit)
+        b = srv.submit(base)
+        time.sleep(1.0)  # a is running on the one worker, b waits behind it
+        assert srv.cancel(b) == 0
+        t0 = time.perf_counter()
+        assert srv.cancel(a) == 1
+        ra = srv.wait(a, timeout=60)
+        stop_s = time.perf_counter() - t0
+        rb = srv.wait(b, timeout=10)
+        assert ra["status"] == 7 and "cancel" in ra["error"] and ra["mean"] is None
+        assert rb["status"] == 7 and rb["run_s"] == 0.0
+        assert stop_s < 5.0 and ra["run_s"] >= 0.9
+        c = srv.submit(base)
+        rc = srv.wait(c, timeout=120)
+        assert rc["status"] == 0 and np.all(np.isfinite(rc["mean"]))
+        assert srv.cancel(c) == 2 and srv.cancel(12345) == 2
