@@ -61,14 +61,14 @@ def make_chain(p, precision, mode, seed=8675309):
     return c
 
 
-def run_chain(c, p, steps, warmup, seed=8675309):
+def run_chain(c, p, steps, warmup, seed=8675309, captured=False):
     """warmup burn-in iterations of dual averaging, then `steps` timed
     post-burn-in iterations, through the C++ host's fit loop (dualgpmf's
     fit_model iterations: eps jitter, HMC update, Gibbs draws, saves every
-    THIN-th iteration)."""
+    THIN-th iteration); captured replays each iteration from a HIP graph."""
     from dualres_amd.sampler import fit_native
     r = fit_native(c, p["Ys"].size, THETA[0], burnin=warmup, nsave=max(1, -(-steps // THIN)), thin=THIN, L=L_STEPS,
-                   seed=seed, post_iters=steps)
+                   seed=seed, post_iters=steps, captured=captured)
     wall = r["wall_s"]
     return {"grad_evals_s": r["grad_evals"] / wall, "wall": wall, "grads": r["grad_evals"],
             "ms_per_iter": 1e3 * wall / r["iterations"], "p50_ms": 1e3 * r["p50_s"], "p99_ms": 1e3 * r["p99_s"],
@@ -363,8 +363,34 @@ def batch_leg(probs, modes=("graph", "streams", "eager"), burnin=30, timed=30, s
     best = max(out["modes"], key=lambda m: out["modes"][m]["post_burnin_grad_evals_per_s"])
     out["best_mode"] = best
     out["post_burnin_grad_evals_per_s"] = out["modes"][best]["post_burnin_grad_evals_per_s"]
+    out["alone"] = alone_study(probs)
+    out["alone"]["overlap_factor"] = round(out["alone"]["sum_ms"] / out["modes"][best]["ms_per_iteration"], 3)
+    log(f"[bench] batch: the chains' iterations one at a time sum to {out['alone']['sum_ms']:.1f} ms, "
+        f"{out['alone']['overlap_factor']:.2f}x the batch's")
     out["tuner"] = tuner_study(probs[:8])
     return out
+
+
+def alone_study(probs, iters=3):
+    """Each chain's HMC iteration (L = 25, eager launches) timed alone on the
+    device, one chain at a time; the sum is what a batch would take with no
+    overlap between chains."""
+    from dualres_amd import Chain
+    ms = []
+    for q in probs:
+        c = Chain(q["ijk"], q["theta"], q["q_hr"], n_datasets=2, precision="fp32", state_mode="compat",
+                  embed=q["embed"], seed=11)
+        c.set_data(q["Yh"], q["Ys"], q["W"])
+        c.add_data_to_mu()
+        c.set_sigma_sq_inv([1.0, 1.0])
+        c.update(0.04, L_STEPS, 0.0)
+        t = time.perf_counter()
+        for _ in range(iters):
+            c.update(0.04, L_STEPS, 0.0)
+        ms.append(1e3 * (time.perf_counter() - t) / iters)
+        c.close()
+    return {"sum_ms": round(float(sum(ms)), 2), "median_ms": round(float(np.median(ms)), 3),
+            "max_ms": round(float(max(ms)), 3), "min_ms": round(float(min(ms)), 3)}
 
 
 def mce_leg(n_images=32, seed=5, nbins=64, maxdist=20.0):
@@ -572,10 +598,18 @@ def main():
             f"mh {r['mh_rate']:.2f}, eps {r['eps']:.3g}")
         results[prec] = r
         if prec == a.precision:
+            main_sum = c.summary_read()[0].copy()
             main_chain = c
             ks, nk = profile_kernels(c, p)
         else:
             c.close()
+    # the same fit with every iteration replayed from a captured HIP graph
+    cc = make_chain(p, a.precision, "compat")
+    rc = run_chain(cc, p, a.steps, a.warmup, captured=True)
+    rc["bitwise_equal_to_eager"] = bool(np.array_equal(cc.summary_read()[0], main_sum))
+    cc.close()
+    log(f"[bench] {a.precision} compat, captured iterations: {rc['grad_evals_s']:.1f} grad evals/s, "
+        f"{rc['ms_per_iter']:.2f} ms/iter, same run bit for bit: {rc['bitwise_equal_to_eager']}")
     rr = None
     if not a.main_only:
         cr = make_chain(p, a.precision, "real")
@@ -629,6 +663,9 @@ def main():
         },
         "fp64_value": round(results["fp64"]["grad_evals_s"], 2) if "fp64" in results else None,
         "fp32_real_state_value": round(rr["grad_evals_s"], 2) if rr else None,
+        "captured_iterations": {"value": round(rc["grad_evals_s"], 2), "ms_per_step": round(rc["ms_per_iter"], 4),
+                                "p50_ms": round(rc["p50_ms"], 3), "p99_ms": round(rc["p99_ms"], 3),
+                                "bitwise_equal_to_eager": rc["bitwise_equal_to_eager"]},
         "iteration_latency_ms": {"p50": round(r["p50_ms"], 3), "p99": round(r["p99_ms"], 3)},
         "kernel_ms_per_leapfrog_step": {k: round(v, 5) for k, v in ks.items()},
         "fft_passes_hbm_gbs": round(fft_bytes / (fft_ms * 1e-3) / 1e9, 1),

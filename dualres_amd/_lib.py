@@ -109,6 +109,7 @@ SIGNATURES = {
     "dr_kriging_build": [_D, C.POINTER(C.c_int), _F, _D, C.POINTER(C.c_int), _F, _D, C.c_double, C.c_int,
                          C.POINTER(_P)],
     "dr_csr_info": [_P, _I64, _I64, _I64, _I64],
+    "dr_csr_solver_counts": [_P, _I64, _I64],
     "dr_csr_copy": [_P, _I64, _I32, _D],
     "dr_csr_free": [_P],
     "dr_mce_offsets": [_I64, _I32],

@@ -9,14 +9,15 @@
 //  3. Unique systems, sorted by size and padded with an identity block to a
 //     multiple of 64, are factorised in batches by a blocked right-looking
 //     float64 Cholesky (k_potrf / k_trsm over 64x64 tiles, the trailing
-//     update k_syrk_mfma on the FP64 matrix cores) and
-//     solved (k_trisolve), then normalised by their sum (kriging_matrix.h:318-321).
+//     update k_syrk_mfma on the FP64 matrix cores) and solved (k_solve), then
+//     normalised by their sum (kriging_matrix.h:318-321).  Systems whose
+//     Cholesky meets a nonpositive pivot (K with its unit diagonal is
+//     indefinite when tau^2 > 1) are factorised again by a blocked LU with
+//     partial pivoting (k_getrf_panel / k_trsm_lu / k_gemm_lu_mfma).
 //  4. Rows gather their system's weights; rows whose weights sum to NaN stay
 //     empty, as in the reference.
 // K and k' entries are float32 (the reference evaluates rbf<float>), taken as
-// the float32 rounding of the float64 kernel.  Systems whose Cholesky hits a
-// nonpositive pivot (indefinite K when tau^2 > 1) are re-solved on the host
-// with partial-pivoting LU.
+// the float32 rounding of the float64 kernel.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -263,29 +264,25 @@ __global__ void __launch_bounds__(256) k_trsm(double* A, int np, int k) {
   }
 }
 
-// Trailing update of the lower tiles, A[BI][BJ] -= P[BI] P[BJ]^T, on the FP64
-// matrix cores: each wavefront owns a 32x32 quadrant of the 64x64 tile as 2x2
-// blocks of 16x16, accumulated by v_mfma_f64_16x16x4f64 over the panel's 64
-// columns, 4 at a time.  Operand layout of the instruction (checked on the
-// device with distinct per-lane values): lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator register j of lane l holds
-// D[4 j + l / 16][l % 16].  Here A = P[BI] (rows of the quadrant, panel
-// columns as k) and B = P[BJ]^T.
+// Trailing updates on the FP64 matrix cores: A[BI][BJ] -= P[BI] Q, where P
+// is the 64-column panel of the tile row BI and Q the 64 x 64 block that
+// multiplies it: P[BJ]^T for the Cholesky (k_syrk_mfma) and the row block of U
+// above BJ for the LU (k_gemm_lu_mfma).  Each wavefront owns a 32x32 quadrant
+// of the 64x64 tile as 2x2 blocks of 16x16, accumulated by
+// v_mfma_f64_16x16x4f64 over the 64 panel columns, 4 at a time.  Operand
+// layout of the instruction (checked on the device with distinct per-lane
+// values): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
+// accumulator register j of lane l holds D[4 j + l / 16][l % 16].
 typedef double dr_v4f64 __attribute__((ext_vector_type(4)));
-__global__ void __launch_bounds__(256) k_syrk_mfma(double* A, int np, int k) {
+template <bool LU>
+__device__ __forceinline__ void tile_update_mfma(double* As, int np, int BI, int BJ, int o) {
   __shared__ double Pi[NB][NB + 1];
-  __shared__ double Pj[NB][NB + 1];
-  double* As = A + (long)blockIdx.x * np * np;
-  const int t = blockIdx.y;
-  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
-  while (bi * (bi + 1) / 2 > t) --bi;
-  const int bj = t - bi * (bi + 1) / 2;
-  const int BI = (k + 1 + bi) * NB, BJ = (k + 1 + bj) * NB, o = k * NB;
+  __shared__ double Pj[NB][NB + 1];  // Pj[col][k]: Q^T
   for (int q = threadIdx.x; q < NB * NB; q += 256) {
     const int i = q / NB, l = q % NB;
     Pi[i][l] = As[(long)(BI + i) * np + o + l];
-    Pj[i][l] = As[(long)(BJ + i) * np + o + l];
+    if (LU) Pj[l][i] = As[(long)(o + i) * np + BJ + l];  // U[o + i][BJ + l]
+    else Pj[i][l] = As[(long)(BJ + i) * np + o + l];
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -319,51 +316,199 @@ __global__ void __launch_bounds__(256) k_syrk_mfma(double* A, int np, int k) {
       }
 }
 
+// Cholesky: lower tiles (bi >= bj) of the trailing matrix, blockIdx.y = the
+// tile's index in the packed lower triangle
+__global__ void __launch_bounds__(256) k_syrk_mfma(double* A, int np, int k) {
+  const int t = blockIdx.y;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  tile_update_mfma<false>(A + (long)blockIdx.x * np * np, np, (k + 1 + bi) * NB, (k + 1 + bj) * NB, k * NB);
+}
+
+// ---------------------------------------------------------------- LU with partial pivoting
+// Systems whose Cholesky meets a nonpositive pivot (the unit-diagonal K with
+// tau^2 > 1 is indefinite) are solved by a blocked right-looking LU with
+// partial pivoting on the device, in the role of the reference's
+// colPivHouseholderQr (kriging_matrix.h:309-313): any factorisation of a
+// nonsingular system gives its solution up to rounding.  Row swaps span the
+// whole row (both factored and trailing columns) and the right-hand side, so
+// the factors satisfy P A = L U and b is permuted in place.  The identity
+// padding stays decoupled: its rows are zero in the data columns and never
+// win a pivot search there.
+
+// one workgroup per system: unblocked LU of the panel of columns [o, o + 64)
+// over rows [o, np); fail = 1 when a pivot column is all zero (singular)
+__global__ void __launch_bounds__(256) k_getrf_panel(double* A, double* bv, int np, int k, int* fail) {
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  double* As = A + (long)blockIdx.x * np * np;
+  double* bs = bv + (long)blockIdx.x * np;
+  const int o = k * NB;
+  for (int j = 0; j < NB; ++j) {
+    const int c = o + j;
+    // pivot: largest |A[i][c]| over i >= c, the first such row on ties
+    double best = -1.0;
+    int bi = c;
+    for (int i = c + (int)threadIdx.x; i < np; i += 256) {
+      const double v = fabs(As[(long)i * np + c]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    rv[threadIdx.x] = best;
+    ri[threadIdx.x] = bi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) {
+        const double v2 = rv[threadIdx.x + st];
+        const int i2 = ri[threadIdx.x + st];
+        if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && i2 < ri[threadIdx.x])) {
+          rv[threadIdx.x] = v2;
+          ri[threadIdx.x] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    const double pmax = rv[0];
+    const int p = ri[0];
+    __syncthreads();  // rv / ri are rewritten by the next column's search
+    if (!(pmax > 0.0)) {  // same value in every thread: a uniform branch
+      if (threadIdx.x == 0) fail[blockIdx.x] = 1;
+      continue;
+    }
+    if (p != c) {
+      for (int t = threadIdx.x; t < np; t += 256) {
+        const double u = As[(long)c * np + t];
+        As[(long)c * np + t] = As[(long)p * np + t];
+        As[(long)p * np + t] = u;
+      }
+      if (threadIdx.x == 0) {
+        const double u = bs[c];
+        bs[c] = bs[p];
+        bs[p] = u;
+      }
+      __syncthreads();
+    }
+    const double d = As[(long)c * np + c];
+    for (int i = c + 1 + (int)threadIdx.x; i < np; i += 256) As[(long)i * np + c] /= d;
+    __syncthreads();
+    const int w = NB - j - 1;
+    if (w > 0) {
+      const long cnt = (long)(np - c - 1) * w;
+      for (long t = threadIdx.x; t < cnt; t += 256) {
+        const int i = c + 1 + (int)(t / w), cc = c + 1 + (int)(t % w);
+        As[(long)i * np + cc] -= As[(long)i * np + c] * As[(long)c * np + cc];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// U12 = L11^-1 A12 for the tile column (k + 1 + blockIdx.y) of the panel's
+// row block (L11 unit lower)
+__global__ void __launch_bounds__(256) k_trsm_lu(double* A, int np, int k) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  double* As = A + (long)blockIdx.x * np * np;
+  const int o = k * NB, c0 = (k + 1 + blockIdx.y) * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    L[i][j] = As[(long)(o + i) * np + o + j];
+    X[i][j] = As[(long)(o + i) * np + c0 + j];
+  }
+  __syncthreads();
+  for (int j = 0; j < NB - 1; ++j) {
+    const int m = NB - j - 1;
+    for (int t = threadIdx.x; t < m * NB; t += 256) {
+      const int i = j + 1 + t / NB, c = t % NB;
+      X[i][c] -= L[i][j] * X[j][c];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, j = t % NB;
+    As[(long)(o + i) * np + c0 + j] = X[i][j];
+  }
+}
+
+// A22 -= L21 U12 over every trailing tile, blockIdx.y = bi * m + bj
+__global__ void __launch_bounds__(256) k_gemm_lu_mfma(double* A, int np, int k, int m) {
+  const int bi = blockIdx.y / m, bj = blockIdx.y % m;
+  tile_update_mfma<true>(A + (long)blockIdx.x * np * np, np, (k + 1 + bi) * NB, (k + 1 + bj) * NB, k * NB);
+}
+
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-// L L^T w = b, then w /= sum(w) (sum 0 -> 1); writes n weights to sol
-__global__ void __launch_bounds__(256) k_trisolve(const double* A, int np, const SysDesc* __restrict__ sd,
-                                                  const double* __restrict__ bv, double* __restrict__ sol,
-                                                  int* __restrict__ nanflag, int s0) {
+// Solves the factored system and normalises: w /= sum(w) (a zero sum counts
+// as 1, kriging_matrix.h:318-321); writes the n weights to sol.  LU = false:
+// L L^T w = b with L the Cholesky factor (lower triangle of A).  LU = true:
+// L U w = b with b already permuted, L unit lower and U upper in A; a system
+// flagged singular gets NaN weights, which drop its rows as a NaN sum does.
+__device__ __forceinline__ double block_dot(double v, double* part) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v = wsum(v);
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  const double t = (part[0] + part[1]) + (part[2] + part[3]);
+  __syncthreads();
+  return t;
+}
+
+template <bool LU>
+__global__ void __launch_bounds__(256) k_solve(const double* A, int np, const SysDesc* __restrict__ sd,
+                                               const double* __restrict__ bv, const int* __restrict__ fail,
+                                               double* __restrict__ sol) {
   extern __shared__ double y[];
   __shared__ double part[4];
   const int s = blockIdx.x;
   const SysDesc d = sd[s];
   const double* As = A + (long)s * np * np;
   const int n = d.n;
+  if (LU && fail[s]) {
+    for (int i = threadIdx.x; i < n; i += 256) sol[d.sol + i] = __builtin_nan("");
+    return;
+  }
   for (int i = threadIdx.x; i < n; i += 256) y[i] = bv[(long)s * np + i];
   __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // forward: row j of L against the solved y[0..j)
   for (int j = 0; j < n; ++j) {
     const double* Lj = As + (long)j * np;
     double acc = 0.0;
     for (int i = threadIdx.x; i < j; i += 256) acc += Lj[i] * y[i];
-    acc = wsum(acc);
-    if (lane == 0) part[w] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) y[j] = (y[j] - ((part[0] + part[1]) + (part[2] + part[3]))) / Lj[j];
+    acc = block_dot(acc, part);
+    if (threadIdx.x == 0) y[j] = LU ? y[j] - acc : (y[j] - acc) / Lj[j];
     __syncthreads();
   }
-  for (int j = n - 1; j >= 0; --j) {
-    const double* Lj = As + (long)j * np;
-    const double wj = y[j] / Lj[j];
-    __syncthreads();
-    if (threadIdx.x == 0) y[j] = wj;
-    for (int i = threadIdx.x; i < j; i += 256) y[i] -= Lj[i] * wj;
-    __syncthreads();
+  if (LU) {  // back: row j of U against the solved y(j..n)
+    for (int j = n - 1; j >= 0; --j) {
+      const double* Uj = As + (long)j * np;
+      double acc = 0.0;
+      for (int i = j + 1 + (int)threadIdx.x; i < n; i += 256) acc += Uj[i] * y[i];
+      acc = block_dot(acc, part);
+      if (threadIdx.x == 0) y[j] = (y[j] - acc) / Uj[j];
+      __syncthreads();
+    }
+  } else {  // back: L^T by columns of L
+    for (int j = n - 1; j >= 0; --j) {
+      const double* Lj = As + (long)j * np;
+      const double wj = y[j] / Lj[j];
+      __syncthreads();
+      if (threadIdx.x == 0) y[j] = wj;
+      for (int i = threadIdx.x; i < j; i += 256) y[i] -= Lj[i] * wj;
+      __syncthreads();
+    }
   }
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) acc += y[i];
-  acc = wsum(acc);
-  if (lane == 0) part[w] = acc;
-  __syncthreads();
-  double tot = (part[0] + part[1]) + (part[2] + part[3]);
+  double tot = block_dot(acc, part);
   if (tot == 0.0) tot = 1.0;
-  if (threadIdx.x == 0 && isnan(tot)) nanflag[s0 + s] = 1;
   for (int i = threadIdx.x; i < n; i += 256) sol[d.sol + i] = y[i] / tot;
 }
 
@@ -397,35 +542,6 @@ struct DBuf {
   void down(T* h, size_t n, hipStream_t s) { DR_HIP(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s)); }
 };
 
-// partial-pivoting LU solve (host fallback for indefinite systems)
-bool lu_solve(std::vector<double>& A, std::vector<double>& b, int n) {
-  std::vector<int> piv(n);
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    double m = std::fabs(A[(long)k * n + k]);
-    for (int i = k + 1; i < n; ++i)
-      if (std::fabs(A[(long)i * n + k]) > m) { m = std::fabs(A[(long)i * n + k]); p = i; }
-    if (m == 0.0) return false;
-    if (p != k) {
-      for (int j = 0; j < n; ++j) std::swap(A[(long)k * n + j], A[(long)p * n + j]);
-      std::swap(b[k], b[p]);
-    }
-    const double d = A[(long)k * n + k];
-    for (int i = k + 1; i < n; ++i) {
-      const double f = A[(long)i * n + k] / d;
-      if (f == 0.0) continue;
-      for (int j = k + 1; j < n; ++j) A[(long)i * n + j] -= f * A[(long)k * n + j];
-      b[i] -= f * b[k];
-    }
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    double s = b[i];
-    for (int j = i + 1; j < n; ++j) s -= A[(long)i * n + j] * b[j];
-    b[i] = s / A[(long)i * n + i];
-  }
-  return true;
-}
-
 float fsq(float a) {
   const volatile float r = a * a;
   return r;
@@ -436,6 +552,7 @@ float fsq(float a) {
 // ---------------------------------------------------------------- host build
 struct KrigingCSR {
   long nrow = 0, ncol = 0, n_unique = 0;
+  long n_lu = 0;  // unique systems solved by the pivoted LU (Cholesky failed)
   std::vector<int64_t> rp;
   std::vector<int32_t> cols;
   std::vector<double> vals;
@@ -640,7 +757,8 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
   for (int i = 0; i < n_rows; ++i)
     DR_REQUIRE(!bad[i], DR_ERUNTIME, "kriging_build: hash collision between distinct kriging systems");
 
-  // unique systems sorted by size, batched by padded size
+  // unique systems sorted by size, batched by padded size; Cholesky first,
+  // then LU with partial pivoting for the systems whose Cholesky failed
   std::vector<int> order(nU);
   std::iota(order.begin(), order.end(), 0);
   std::sort(order.begin(), order.end(), [&](int a, int b) { return cnt[urow[a]] > cnt[urow[b]]; });
@@ -650,88 +768,73 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
   DBuf<double> d_sol(solsz);
   DBuf<float> d_K((size_t)nP * nP);
   d_K.up(K.data(), (size_t)nP * nP, s);
-  std::vector<int> failed_u, nanf(nU, 0);
-  DBuf<int> d_nan(nU);
-  DR_HIP(hipMemsetAsync(d_nan.p, 0, nU * sizeof(int), s));
   const double budget = 6.0e9;  // bytes of factor scratch per batch
-  size_t pos = 0;
   DBuf<double> d_A, d_b;
   DBuf<SysDesc> d_sd;
   DBuf<int> d_fail;
   size_t capA = 0, capB = 0, capS = 0;
-  while (pos < (size_t)nU) {
-    const int n0 = cnt[urow[order[pos]]];
-    const int np = ((n0 + NB - 1) / NB) * NB;
-    const long per = (long)np * np * 8;
-    const int bmax = (int)std::max<long>(1, std::min<long>(4096, (long)(budget / per)));
-    std::vector<SysDesc> sd;
-    std::vector<int> bu;
-    while (pos < (size_t)nU && (int)sd.size() < bmax) {
-      const int u = order[pos];
-      sd.push_back(SysDesc{rp[urow[u]], cnt[urow[u]], usol[u]});
-      bu.push_back(u);
-      ++pos;
-    }
-    const int B = (int)sd.size();
-    if ((size_t)B * np * np > capA) { capA = (size_t)B * np * np; d_A.alloc(capA); }
-    if ((size_t)B * np > capB) { capB = (size_t)B * np; d_b.alloc(capB); }
-    if ((size_t)B > capS) { capS = B; d_sd.alloc(capS); d_fail.alloc(capS); }
-    d_sd.up(sd.data(), B, s);
-    DR_HIP(hipMemsetAsync(d_fail.p, 0, B * sizeof(int), s));
-    const int gx = std::max(1, std::min(cdiv((long)np * np, 256), 256));
-    hipLaunchKernelGGL(k_assemble, dim3(gx, B), dim3(256), 0, s, d_sd.p, np, d_K.p, nP, d_pidx.p, d_kp.p, d_A.p,
-                       d_b.p);
-    DR_HIP(hipGetLastError());
-    const int nt = np / NB;
-    for (int k = 0; k < nt; ++k) {
-      hipLaunchKernelGGL(k_potrf, dim3(B), dim3(256), 0, s, d_A.p, np, k, d_fail.p);
-      const int m = nt - k - 1;
-      if (m > 0) {
-        hipLaunchKernelGGL(k_trsm, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
-        hipLaunchKernelGGL(k_syrk_mfma, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
+  auto solve_all = [&](const std::vector<int>& uids, bool lu) {
+    std::vector<int> failed;
+    size_t pos = 0;
+    while (pos < uids.size()) {
+      const int n0 = cnt[urow[uids[pos]]];
+      const int np = ((n0 + NB - 1) / NB) * NB;
+      const long per = (long)np * np * 8;
+      const int bmax = (int)std::max<long>(1, std::min<long>(4096, (long)(budget / per)));
+      std::vector<SysDesc> sd;
+      std::vector<int> bu;
+      while (pos < uids.size() && (int)sd.size() < bmax) {
+        const int u = uids[pos];
+        sd.push_back(SysDesc{rp[urow[u]], cnt[urow[u]], usol[u]});
+        bu.push_back(u);
+        ++pos;
       }
+      const int B = (int)sd.size();
+      if ((size_t)B * np * np > capA) { capA = (size_t)B * np * np; d_A.alloc(capA); }
+      if ((size_t)B * np > capB) { capB = (size_t)B * np; d_b.alloc(capB); }
+      if ((size_t)B > capS) { capS = B; d_sd.alloc(capS); d_fail.alloc(capS); }
+      d_sd.up(sd.data(), B, s);
+      DR_HIP(hipMemsetAsync(d_fail.p, 0, B * sizeof(int), s));
+      const int gx = std::max(1, std::min(cdiv((long)np * np, 256), 256));
+      hipLaunchKernelGGL(k_assemble, dim3(gx, B), dim3(256), 0, s, d_sd.p, np, d_K.p, nP, d_pidx.p, d_kp.p, d_A.p,
+                         d_b.p);
       DR_HIP(hipGetLastError());
-    }
-    std::vector<int> fail(B);
-    hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p, d_sol.p,
-                       d_nan.p, 0);
-    DR_HIP(hipGetLastError());
-    (void)bu;
-    d_fail.down(fail.data(), B, s);
-    DR_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < B; ++i)
-      if (fail[i]) failed_u.push_back(bu[i]);
-  }
-  // indefinite systems: host LU
-  if (!failed_u.empty()) {
-    std::vector<int> pidx_h(nnz);
-    std::vector<float> kp_h(nnz);
-    d_pidx.down(pidx_h.data(), nnz, s);
-    d_kp.down(kp_h.data(), nnz, s);
-    DR_HIP(hipStreamSynchronize(s));
-    for (int u : failed_u) {
-      const long off = rp[urow[u]];
-      const int n = cnt[urow[u]];
-      std::vector<double> A((size_t)n * n), b(n);
-      for (int i = 0; i < n; ++i) {
-        b[i] = kp_h[off + i];
-        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = K[(size_t)pidx_h[off + i] * nP + pidx_h[off + j]];
+      const int nt = np / NB;
+      for (int k = 0; k < nt; ++k) {
+        const int m = nt - k - 1;
+        if (lu) {
+          hipLaunchKernelGGL(k_getrf_panel, dim3(B), dim3(256), 0, s, d_A.p, d_b.p, np, k, d_fail.p);
+          if (m > 0) {
+            hipLaunchKernelGGL(k_trsm_lu, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
+            hipLaunchKernelGGL(k_gemm_lu_mfma, dim3(B, m * m), dim3(256), 0, s, d_A.p, np, k, m);
+          }
+        } else {
+          hipLaunchKernelGGL(k_potrf, dim3(B), dim3(256), 0, s, d_A.p, np, k, d_fail.p);
+          if (m > 0) {
+            hipLaunchKernelGGL(k_trsm, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
+            hipLaunchKernelGGL(k_syrk_mfma, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
+          }
+        }
+        DR_HIP(hipGetLastError());
       }
-      std::vector<double> w(n, std::nan(""));
-      if (lu_solve(A, b, n)) w = b;
-      double sum = 0.0;
-      for (double v : w) sum += v;
-      if (sum == 0.0) sum = 1.0;
-      for (double& v : w) v /= sum;
-      DR_HIP(hipMemcpyAsync(d_sol.p + usol[u], w.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+      if (lu)
+        hipLaunchKernelGGL(k_solve<true>, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
+                           d_fail.p, d_sol.p);
+      else
+        hipLaunchKernelGGL(k_solve<false>, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
+                           d_fail.p, d_sol.p);
+      DR_HIP(hipGetLastError());
+      std::vector<int> fail(B);
+      d_fail.down(fail.data(), B, s);
       DR_HIP(hipStreamSynchronize(s));
-      int nf = std::isnan(sum) ? 1 : 0;
-      DR_HIP(hipMemcpyAsync(d_nan.p + u, &nf, sizeof(int), hipMemcpyHostToDevice, s));
-      DR_HIP(hipStreamSynchronize(s));
+      if (!lu)
+        for (int i = 0; i < B; ++i)
+          if (fail[i]) failed.push_back(bu[i]);
     }
-  }
-  // NaN systems are stored at the position their batch had; map back by unique id
-  // (k_trisolve flags by batch slot, so re-derive per unique from the solution)
+    return failed;
+  };
+  const std::vector<int> indefinite = solve_all(order, false);
+  if (!indefinite.empty()) (void)solve_all(indefinite, true);
   std::vector<int> uidd(n_rows);
   for (int i = 0; i < n_rows; ++i) uidd[i] = uid[i] < 0 ? 0 : uid[i];
   d_uid.up(uidd.data(), n_rows, s);
@@ -751,6 +854,7 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
   out.nrow = n_rows;
   out.ncol = nnz_hr;
   out.n_unique = nU;
+  out.n_lu = (long)indefinite.size();
   out.rp.assign(n_rows + 1, 0);
   out.cols.reserve(nnz);
   out.vals.reserve(nnz);
@@ -767,7 +871,6 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
     out.rp[i + 1] = (int64_t)out.cols.size();
   }
   (void)oobv;
-  (void)nanf;
   return out;
 }
 
@@ -796,6 +899,14 @@ int dr_csr_info(dr_csr* m, int64_t* nrow, int64_t* ncol, int64_t* nnz, int64_t* 
     *ncol = m->m.ncol;
     *nnz = (int64_t)m->m.cols.size();
     if (n_unique) *n_unique = m->m.n_unique;
+  });
+}
+
+int dr_csr_solver_counts(dr_csr* m, int64_t* n_cholesky, int64_t* n_lu) {
+  return drh::guarded([&] {
+    DR_REQUIRE(m && n_cholesky && n_lu, DR_ELOGIC, "dr_csr_solver_counts: null argument");
+    *n_lu = m->m.n_lu;
+    *n_cholesky = m->m.n_unique - m->m.n_lu;
   });
 }
 

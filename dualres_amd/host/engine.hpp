@@ -110,6 +110,12 @@ class Chain {
     check(dr_update(h_, eps, L, u, &r));
     return r;
   }
+  // the same iteration replayed from a graph captured once per L
+  dr_update_result update_captured(double eps, int L, double u) {
+    dr_update_result r{};
+    check(dr_update_captured(h_, eps, L, u, &r));
+    return r;
+  }
   std::vector<double> mu_() {
     std::vector<double> out(n_out());
     check(dr_get_mu_output(h_, out.data(), (int64_t)out.size()));

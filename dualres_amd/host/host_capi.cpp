@@ -71,7 +71,7 @@ int drh_hmc_trace(int burnin, int n_save, int thin, int integrator_steps, double
 namespace {
 template <typename T>
 void fit_chain(dr_chain* c, int nd, int64_t n_h, int64_t n_s, int64_t n_out, double tau2, int burnin, int nsave,
-               int thin, int L, uint64_t seed, int64_t post_iters, double* out) {
+               int thin, int L, uint64_t seed, int64_t post_iters, int captured, double* out) {
   using namespace drhost;
   rng().seed((std::mt19937::result_type)seed);
   Chain chain(c, nd, (long)n_h, (long)n_out);
@@ -85,10 +85,15 @@ void fit_chain(dr_chain* c, int nd, int64_t n_h, int64_t n_s, int64_t n_out, dou
   chain.set_sigma_sq_inv(sig);
   FitOptions opt;
   opt.quiet = true;
+  opt.captured = captured != 0;
   NullStream quiet;
   std::ofstream none;
   std::vector<double> lat;
-  McmcSummary<T> summ = sample_chain<T>(chain, (long)n_h, (long)n_s, hmc, sig, none, opt, quiet, &lat, (long)post_iters);
+  std::vector<int> steps;
+  McmcSummary<T> summ =
+      sample_chain<T>(chain, (long)n_h, (long)n_s, hmc, sig, none, opt, quiet, &lat, (long)post_iters, &steps);
+  double grads = 0;
+  for (int l : steps) grads += l + 1;
   double wall = 0;
   for (double t : lat) wall += t;
   std::vector<double> sorted = lat;
@@ -101,7 +106,7 @@ void fit_chain(dr_chain* c, int nd, int64_t n_h, int64_t n_s, int64_t n_out, dou
     return i + 1 < sorted.size() ? sorted[i] * (1 - f) + sorted[i + 1] * f : sorted[i];
   };
   out[0] = wall;
-  out[1] = (double)lat.size() * (L + 1);
+  out[1] = grads;
   out[2] = hmc.metropolis_hastings_rate();
   out[3] = hmc.eps_value();
   out[4] = pct(0.5);
@@ -117,15 +122,18 @@ extern "C" {
 // chain created elsewhere with its data set, mu holding the data and its
 // output indices set.  precision 0 runs the host scalars in double as
 // dualgpm, 1 in float as dualgpmf.  post_iters > 0 stops after that many
-// post-burn-in iterations.  out[7]: post-burn-in wall seconds, gradient
-// evaluations, MH rate, final eps, p50 and p99 iteration seconds, iterations.
+// post-burn-in iterations; captured != 0 replays each iteration from a
+// captured graph.  out[7]: post-burn-in wall seconds, gradient
+// evaluations (the sum of L + 1 over the iterations), MH rate, final eps,
+// p50 and p99 iteration seconds, iterations.
 int drh_fit_chain(dr_chain* c, int precision, int n_datasets, int64_t n_h, int64_t n_s, int64_t n_out, double tau2,
-                  int burnin, int nsave, int thin, int L, uint64_t seed, int64_t post_iters, double* out) {
+                  int burnin, int nsave, int thin, int L, uint64_t seed, int64_t post_iters, int captured,
+                  double* out) {
   try {
     if (precision == 0)
-      fit_chain<double>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, out);
+      fit_chain<double>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, captured, out);
     else
-      fit_chain<float>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, out);
+      fit_chain<float>(c, n_datasets, n_h, n_s, n_out, tau2, burnin, nsave, thin, L, seed, post_iters, captured, out);
     return 0;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "drh_fit_chain: %s\n", e.what());

@@ -178,6 +178,9 @@ struct FitOptions {
   // set by another thread to end the fit: checked before the kriging build
   // and before every HMC iteration, which then throws Cancelled
   const std::atomic<bool>* cancel = nullptr;
+  // HMC iterations replayed from captured graphs (dr_update_captured; the
+  // results are the same bit for bit)
+  bool captured = false;
 };
 
 struct Cancelled : std::runtime_error {
@@ -199,12 +202,14 @@ struct MultiResData {  // MultiResData.h:21-127
 // prepared chain (data, mu with data added, output indices, initial sigma
 // draws in sig): HMC update, Gibbs draws (and tau with opt.update_tau), dual
 // averaging and saves.  iter_s, when given, receives the wall seconds of
-// every post-burn-in iteration; stop_after > 0 ends the loop after that many
-// post-burn-in iterations.
+// every post-burn-in iteration and iter_L its leapfrog steps (the first
+// post-burn-in iteration runs one step when burn-in ended before the
+// step-size search did, as in the reference); stop_after > 0 ends the loop
+// after that many post-burn-in iterations.
 template <typename T>
 McmcSummary<T> sample_chain(Chain& chain, long n_h, long n_s, HMCParameters<T>& hmc, std::vector<double> sig,
                             std::ofstream& samples_out, const FitOptions& opt, std::ostream& out,
-                            std::vector<double>* iter_s, long stop_after = -1) {
+                            std::vector<double>* iter_s, long stop_after = -1, std::vector<int>* iter_L = nullptr) {
   const int nd = chain.n_datasets();
   McmcSummary<T> summ(chain.n_out(), nd);
   int save_count = 0;
@@ -221,7 +226,7 @@ McmcSummary<T> sample_chain(Chain& chain, long n_h, long n_s, HMCParameters<T>& 
     const T eps = hmc.eps();
     const int L = hmc.integrator_steps();
     const double u = U01(rng());
-    const dr_update_result r = chain.update(eps, L, u);
+    const dr_update_result r = opt.captured ? chain.update_captured(eps, L, u) : chain.update(eps, L, u);
     if (opt.monitor)
       out << std::setprecision(4) << "Energy Transition (" << r.H0 << " -> " << r.H1 << ");  step size ("
                 << eps << ");  MH = (" << r.R << ")" << std::endl;
@@ -261,6 +266,7 @@ McmcSummary<T> sample_chain(Chain& chain, long n_h, long n_s, HMCParameters<T>& 
     }
     if (post && iter_s)
       iter_s->push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count());
+    if (post && iter_L) iter_L->push_back(L);
     if (post && stop_after > 0 && --stop_after == 0) break;
   }
   const auto stop = std::chrono::high_resolution_clock::now();

@@ -32,9 +32,11 @@ def kriging_build(hr, q_hr, std, q_std, theta, radius, device=0, return_info=Fal
         cols = np.empty(nnz.value, dtype=np.int32)
         vals = np.empty(nnz.value, dtype=np.float64)
         L.check(lib.dr_csr_copy(h, L.iptr64(rp), L.iptr32(cols), L.dptr(vals)))
+        nch, nlu = C.c_int64(), C.c_int64()
+        L.check(lib.dr_csr_solver_counts(h, C.byref(nch), C.byref(nlu)))
     finally:
         lib.dr_csr_free(h)
     W = sp.csr_matrix((vals, cols, rp), shape=(nrow.value, ncol.value))
     if return_info:
-        return W, {"n_unique_solves": nu.value, "nnz": nnz.value}
+        return W, {"n_unique_solves": nu.value, "nnz": nnz.value, "n_cholesky": nch.value, "n_lu": nlu.value}
     return W

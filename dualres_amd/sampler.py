@@ -227,12 +227,13 @@ def fit(chain, hmc, tau2, n_s=0, seed=1, on_iteration=None, update_tau=False):
                   "post_burnin_grad_evals": grads}
 
 
-def fit_native(chain, n_s, tau2, burnin, nsave, thin, L, seed, post_iters=0):
+def fit_native(chain, n_s, tau2, burnin, nsave, thin, L, seed, post_iters=0, captured=False):
     """The C++ host's fit loop (dualgpm's fit_model iterations, in
     libdualres_host.so) on a prepared chain: the sigma initialisation, HMC
     updates, Gibbs draws, dual averaging and device summaries, with the host
     scalars in the chain's precision as dualgpm (float64) / dualgpmf
-    (float32) keep them.  Returns post-burn-in timing and sampler stats."""
+    (float32) keep them; captured replays each iteration from a captured
+    graph.  Returns post-burn-in timing and sampler stats."""
     import ctypes as C
     import os
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdualres_host.so")
@@ -240,12 +241,12 @@ def fit_native(chain, n_s, tau2, burnin, nsave, thin, L, seed, post_iters=0):
         raise ImportError(f"{path} is not built (run __graft_entry__.build())")
     lib = C.CDLL(path)
     lib.drh_fit_chain.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int,
-                                  C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.POINTER(C.c_double)]
+                                  C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     lib.drh_fit_chain.restype = C.c_int
     out = (C.c_double * 7)()
     rc = lib.drh_fit_chain(chain._h, 0 if chain.precision == "fp64" else 1, chain.n_datasets, chain.n_h, int(n_s),
                            chain.n_out, float(tau2), int(burnin), int(nsave), int(thin), int(L), int(seed),
-                           int(post_iters), out)
+                           int(post_iters), int(bool(captured)), out)
     if rc != 0:
         raise RuntimeError("drh_fit_chain failed")
     return {"wall_s": out[0], "grad_evals": int(out[1]), "mh_rate": out[2], "eps": out[3], "p50_s": out[4],

@@ -258,6 +258,11 @@ int dr_kriging_build(const double* hr, const int hr_dims[3], const float q_hr[16
                      const double theta[3], double radius, int device, dr_csr** out);
 int dr_csr_info(dr_csr* m, int64_t* nrow, int64_t* ncol, int64_t* nnz, int64_t* n_unique_solves);
 int dr_csr_copy(dr_csr* m, int64_t* rowptr, int32_t* cols, double* vals);
+/* Unique systems solved by Cholesky and by the pivoted LU that takes the
+ * indefinite ones (tau^2 > 1 with K's unit diagonal).  No counterpart in the
+ * reference, whose colPivHouseholderQr takes every system
+ * (kriging_matrix.h:309-313); a diagnostic of which solver ran. */
+int dr_csr_solver_counts(dr_csr* m, int64_t* n_cholesky, int64_t* n_lu);
 int dr_csr_free(dr_csr* m);
 
 /* --- minimum contrast summary (estimate_kernel_parameters.h:53-101) ------- */

@@ -42,11 +42,15 @@ int drh_hmc_trace(int burnin, int n_save, int thin, int integrator_steps, double
  * output indices set: HMC updates, Gibbs draws, dual averaging and device
  * summaries.  precision 0 keeps the host scalars in double (dualgpm), 1 in
  * float (dualgpmf).  post_iters > 0 stops after that many post-burn-in
- * iterations.  out[7]: post-burn-in wall seconds, gradient evaluations, MH
- * rate, final eps, p50 and p99 iteration seconds, iterations.  Returns 1 on
- * any error (printed to stderr). */
+ * iterations.  captured != 0 runs each iteration through dr_update_captured
+ * (the same results bit for bit).  out[7]: post-burn-in wall seconds, gradient evaluations (the
+ * sum of L + 1 over those iterations: the first runs one step when burn-in
+ * ended before the step-size search did), MH rate, final eps, p50 and p99
+ * iteration seconds, iterations.  Returns 1 on any error (printed to
+ * stderr). */
 int drh_fit_chain(dr_chain* c, int precision, int n_datasets, int64_t n_h, int64_t n_s, int64_t n_out, double tau2,
-                  int burnin, int nsave, int thin, int L, uint64_t seed, int64_t post_iters, double* out);
+                  int burnin, int nsave, int thin, int L, uint64_t seed, int64_t post_iters, int captured,
+                  double* out);
 
 /* NIfTI-1 read (nifti_manipulation.inl's image loading): dims[3] and the
  * row-major qform[16] out, data (nx*ny*nz, x fastest) when non-null.

@@ -517,24 +517,29 @@ def test_kriging_pattern_flat_offset_wrap(gpu_available):
     assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
 
 
-def test_kriging_indefinite_systems_tau2_above_one(gpu_available):
+@pytest.mark.parametrize("dims,vox", [((14, 12, 10), 2.0), ((18, 16, 14), 1.0)])
+def test_kriging_indefinite_systems_tau2_above_one(gpu_available, dims, vox):
     """tau^2 = 1.5 with the reference's unit diagonal (kriging_matrix.h:118)
     makes K indefinite, so the device Cholesky meets a nonpositive pivot and
-    those systems are solved again with partial pivoting: pattern bit-exact
-    and weights within 1e-6 of the oracle's pivoted float64 solves
-    (kriging_matrix.h:309-313 solves the same systems by column-pivoted QR)."""
+    those systems are factorised again by the device LU with partial
+    pivoting: pattern bit-exact and weights within 1e-6 of the oracle's
+    pivoted float64 solves (kriging_matrix.h:309-313 solves the same systems
+    by column-pivoted QR).  At 2 mm the systems fit one 64-tile (panel
+    factorisation only); at 1 mm they have 437 unknowns, 7 tiles, so the
+    blocked triangular solve and the MFMA trailing update run too."""
     from dualres_amd import problem
     from dualres_amd.kriging import kriging_build
     from oracle import kriging
     theta = (1.5, 0.132, 1.5)
-    p = problem.dual_problem((14, 12, 10), 2.0, theta, seed_gp=19)
-    radius = 2.3 * 2.0
+    p = problem.dual_problem(dims, vox, theta, seed_gp=19)
+    radius = 4.6
     P = kriging.neighborhood_perturbation(p["q_hr"], radius)
     K = kriging.perturbation_kernel_distances(P, p["q_hr"], theta).astype(np.float64)
     assert np.linalg.eigvalsh(K).min() < 0          # indefinite
     rp, cols, vals, nrow, ncol, st = kriging.get_sparse_kriging_matrix_data(
         p["hr"], p["std"], p["q_hr"], p["q_std"], theta, radius)
-    W = kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], theta, radius)
+    W, info = kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], theta, radius, return_info=True)
+    assert info["n_lu"] > 0 and info["n_lu"] + info["n_cholesky"] == info["n_unique_solves"]
     assert np.array_equal(W.indptr, rp) and np.array_equal(W.indices, cols)
     assert np.max(np.abs(W.data - vals)) <= 1e-6 * max(1.0, np.max(np.abs(vals)))
     assert np.allclose(np.asarray(W.sum(axis=1)).ravel()[np.diff(rp) > 0], 1.0, atol=1e-12)
@@ -548,15 +553,21 @@ def test_native_fit_loop_on_a_prepared_chain(gpu_available):
     from dualres_amd.sampler import fit_native
     p = small_dual(dims=(16, 14, 12))
     outs = []
-    for _ in range(2):
+    for captured in (False, False, True):
         c = engine_chain(p, THETA2, precision="fp32", state_mode="compat", seed=6)
         c.add_data_to_mu()
         c.set_output_indices(p["ijk"])
-        r = fit_native(c, p["Ys"].size, THETA2[0], burnin=20, nsave=10, thin=3, L=6, seed=12, post_iters=25)
+        r = fit_native(c, p["Ys"].size, THETA2[0], burnin=20, nsave=10, thin=3, L=6, seed=12, post_iters=25,
+                       captured=captured)
         outs.append((r, c.mu(), c.summary_read()))
         c.close()
-    (r1, m1, s1), (r2, m2, s2) = outs
-    assert r1["iterations"] == 25 and r1["grad_evals"] == 25 * 7
+    (r1, m1, s1), (r2, m2, s2), (r3, m3, s3) = outs
+    # iterations replayed from captured graphs give the same run bit for bit
+    assert r3["mh_rate"] == r1["mh_rate"] and r3["eps"] == r1["eps"] and r3["grad_evals"] == r1["grad_evals"]
+    assert np.array_equal(m3, m1) and np.array_equal(s3[0], s1[0]) and np.array_equal(s3[1], s1[1])
+    # every timed iteration runs L = 6 steps, except a first one of 1 step
+    # when burn-in ends before the step-size search does (as in the reference)
+    assert r1["iterations"] == 25 and 25 * 7 - 5 <= r1["grad_evals"] <= 25 * 7
     assert 0.0 < r1["mh_rate"] <= 1.0 and r1["eps"] != 1.0 / 6
     assert r1["mh_rate"] == r2["mh_rate"] and r1["eps"] == r2["eps"]
     assert np.array_equal(m1, m2) and np.array_equal(s1[0], s2[0]) and s1[3] == 8
