@@ -225,6 +225,7 @@ def serving_leg(rates=(1.0, 2.0, 4.0), n_req=32, nsave=50, burnin=50, workers=0,
     from dualres_amd import serve
     reqs = serving_requests(n_req, nsave, burnin, seed)
     rng = np.random.default_rng(seed)
+    every = max(1, min(50, n_req // 10))
     out = []
     for rate in rates:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
@@ -236,12 +237,12 @@ def serving_leg(rates=(1.0, 2.0, 4.0), n_req=32, nsave=50, burnin=50, workers=0,
                 if dt > 0:
                     time.sleep(dt)
                 ids.append(srv.submit(r))
-                if (k + 1) % 50 == 0:
+                if (k + 1) % every == 0:
                     log(f"[bench] serving at {rate}/s: {k + 1} of {n_req} submitted")
             res = []
             for k, i in enumerate(ids):
                 res.append(srv.wait(i, timeout=1200))
-                if (k + 1) % 50 == 0:
+                if (k + 1) % every == 0:
                     log(f"[bench] serving at {rate}/s: {k + 1} of {n_req} done")
             wall = time.perf_counter() - t0
         bad = [x["error"] for x in res if x["status"] != 0]
