@@ -487,6 +487,63 @@ def host_cores():
             "threads_80pct": max(1, threads * 4 // 5)}
 
 
+CFG1_THETA = (0.806, 0.132, 1.0)
+CFG1_EMBED = (80, 96, 80)
+
+
+def config1_problem():
+    """The plain single-resolution workload: 40x48x40 at 2 mm, ellipsoid
+    mask, one GP draw (tau2 = 0.806, psi = 0.132, nu = 1) + N(0, 1) noise."""
+    from dualres_amd import problem
+    return problem.single_problem((40, 48, 40), 2.0, CFG1_THETA, seed_gp=20261202, seed_noise=1)
+
+
+def config1_leg(seed=8675309):
+    """A whole single-resolution fit at the 2 mm workload, as dualgpm runs it:
+    FP64, embedding 80x96x80, L = 25, 200 burn-in + 200 saved iterations
+    (thin 1) through the C++ fit loop; wall time of the whole fit and the
+    post-burn-in grad evals/s."""
+    from dualres_amd import Chain
+    from dualres_amd.sampler import fit_native
+    p = config1_problem()
+    c = Chain(p["ijk"], CFG1_THETA, p["q_hr"], n_datasets=1, precision="fp64", state_mode="compat",
+              embed=CFG1_EMBED, seed=seed)
+    c.set_data(p["Yh"])
+    c.add_data_to_mu()
+    c.set_output_indices(p["ijk"])
+    t = time.perf_counter()
+    r = fit_native(c, 0, CFG1_THETA[0], burnin=200, nsave=200, thin=1, L=L_STEPS, seed=seed)
+    wall = time.perf_counter() - t
+    c.close()
+    out = {"workload": "single-res 40x48x40 @2mm, embedding 80x96x80, fp64, L=25, 200 burn-in + 200 saved",
+           "fit_wall_s": round(wall, 3), "post_burnin_grad_evals_per_s": round(r["grad_evals"] / r["wall_s"], 1),
+           "post_burnin_ms_per_iteration": round(1e3 * r["wall_s"] / r["iterations"], 3), "mh_rate": r["mh_rate"],
+           "eps": r["eps"]}
+    log(f"[bench] config 1: whole fit {wall:.2f}s, {out['post_burnin_grad_evals_per_s']:.0f} post-burn-in "
+        f"grad evals/s, {out['post_burnin_ms_per_iteration']:.2f} ms per iteration")
+    return out
+
+
+def cpu_config1_chain(eps, threads, iters=400):
+    """The C++ CPU restatement running the 2 mm workload's 400 HMC iterations
+    (L = 25, FP64) at a fixed step size on `threads` threads."""
+    from oracle import cpu_port
+    p = config1_problem()
+    c = cpu_port.CpuChain(p["ijk"], CFG1_THETA, p["q_hr"], CFG1_EMBED, p["Yh"], precision="fp64", threads=threads,
+                          seed=42)
+    c.add_data_to_mu()
+    c.set_sigma_sq_inv([1.0])
+    t = time.perf_counter()
+    for i in range(iters):
+        c.update(eps, L_STEPS)
+        if (i + 1) % 50 == 0:
+            log(f"[bench] cpu config 1: {i + 1} of {iters} iterations, {time.perf_counter() - t:.1f}s")
+    wall = time.perf_counter() - t
+    c.close()
+    return {"iterations": iters, "threads": threads, "wall_s": round(wall, 2),
+            "grad_evals_s": round(iters * (L_STEPS + 1) / wall, 2)}
+
+
 def cpu_iterations(p, precision, threads, w_threads, iters, eps, tag):
     """Seconds of each of `iters` HMC iterations (L = 25, momentum, energies,
     accept, Gibbs draws) of the C++ restatement of the reference's sampler
@@ -533,7 +590,8 @@ def cpu_baseline(p, eps):
 def cpu_study(p, eps, iters=3):
     """Medians of `iters` iterations: float32 and float64, all threads and 80%
     of them (the reference's default share, defines.h:73-74) with W on one
-    thread, and all threads with OpenMP W."""
+    thread, and all threads with OpenMP W; then a whole 400-iteration chain
+    at the 2 mm single-resolution workload."""
     hc = host_cores()
     runs = []
     for prec in ("fp32", "fp64"):
@@ -542,7 +600,10 @@ def cpu_study(p, eps, iters=3):
             med = float(np.median(ts))
             runs.append({"precision": prec, "threads": threads, "w_threads": w, "iteration_s": [round(x, 2) for x in ts],
                          "median_s": round(med, 2), "grad_evals_s": round((L_STEPS + 1) / med, 3)})
-    return {"host": hc, "sample": f"median of {iters} {CPU_SAMPLE}", "eps": eps, "runs": runs}
+    cfg1 = cpu_config1_chain(eps=0.1, threads=hc["threads"])
+    return {"host": hc, "sample": f"median of {iters} {CPU_SAMPLE}", "eps": eps, "runs": runs,
+            "config1_chain": dict(cfg1, sample="400 HMC iterations (L = 25, FP64, fixed step size 0.1) of the same "
+                                               "restatement at the 2 mm single-resolution workload")}
 
 
 def main():
@@ -568,6 +629,9 @@ def main():
     ap.add_argument("--cpu-study", action="store_true",
                     help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
                          "medians of 3 iterations) and print its JSON")
+    ap.add_argument("--config1-only", action="store_true",
+                    help="run only the 2 mm single-resolution fit on the GPU and the CPU restatement's 400-iteration "
+                         "chain at the same workload, and print their JSON")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving leg and print its JSON (latency study with --serve-requests)")
     a = ap.parse_args()
@@ -578,6 +642,11 @@ def main():
     if a.serve_only:
         print(json.dumps(serving_leg(rates=tuple(float(x) for x in a.serve_rates.split(",")),
                                      n_req=a.serve_requests, nsave=a.serve_nsave, burnin=a.serve_nsave)), flush=True)
+        return
+    if a.config1_only:
+        g = config1_leg()
+        cpu = cpu_config1_chain(eps=0.1, threads=host_cores()["threads"])
+        print(json.dumps({"gpu": g, "cpu": cpu, "host": host_cores()}), flush=True)
         return
     p = build_problem()
     if a.cpu_study:
@@ -637,6 +706,7 @@ def main():
     batch = None if a.main_only or a.no_serve else batch_leg(batch_problems(), modes=tuple(a.batch_modes.split(",")),
                                                              tune=a.tune)
     estimation = None if a.main_only or a.no_serve else mce_leg(a.mce_images)
+    config1 = None if a.main_only else config1_leg()
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(p, eps=results[a.precision]["eps"])
@@ -676,6 +746,7 @@ def main():
         "serving_latency": serving,
         "batch_throughput": batch,
         "parameter_estimation": estimation,
+        "config1_fit": config1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1044,6 +1044,40 @@ struct Chain : public ChainBase {
     // cumulative over all data voxels and untouched voxels add nothing.
     for (size_t k = 0; k < tfp_start.size(); ++k)
       DR_REQUIRE(tfp_start[k] == tfp[k], DR_ELOGIC, "W stencil: far transposed ranges not contiguous");
+    // 5b. touched voxels by decreasing entry count: the kWtvLanes-lane groups
+    //     of a wavefront then run loops of about the same length (entry
+    //     counts range from 1 to ~140 at 1 mm; in voxel order a wavefront
+    //     waits on its longest voxel).  Each voxel keeps its entry list.
+    {
+      const size_t nt = tg.size();
+      std::vector<size_t> ord(nt);
+      std::iota(ord.begin(), ord.end(), (size_t)0);
+      auto len = [&](size_t k) { return (long)(trp[k + 1] - trp[k]) + (long)(tfp[k + 1] - tfp[k]); };
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return len(x) > len(y); });
+      std::vector<uint16_t> td2;
+      std::vector<T> tw2, tfw2;
+      std::vector<int32_t> tg2, tb2, trp2{0}, tfp2{0}, tfi2;
+      td2.reserve(tdall.size());
+      tw2.reserve(twall.size());
+      for (size_t o : ord) {
+        tg2.push_back(tg[o]);
+        tb2.push_back(tbase[o]);
+        td2.insert(td2.end(), tdall.begin() + trp[o], tdall.begin() + trp[o + 1]);
+        tw2.insert(tw2.end(), twall.begin() + trp[o], twall.begin() + trp[o + 1]);
+        trp2.push_back((int32_t)td2.size());
+        tfi2.insert(tfi2.end(), tfidx.begin() + tfp[o], tfidx.begin() + tfp[o + 1]);
+        tfw2.insert(tfw2.end(), tfw.begin() + tfp[o], tfw.begin() + tfp[o + 1]);
+        tfp2.push_back((int32_t)tfi2.size());
+      }
+      tg.swap(tg2);
+      tbase.swap(tb2);
+      tdall.swap(td2);
+      twall.swap(tw2);
+      trp.swap(trp2);
+      tfidx.swap(tfi2);
+      tfw.swap(tfw2);
+      tfp.swap(tfp2);
+    }
     // 6. every device access of the four kernels stays inside its buffer
     {
       long fmin = 0, fmax = 0;

@@ -453,7 +453,7 @@ __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 
 // lanes per voxel of the boundary W^T v part (the transposed entry lists are
 // interleaved in blocks of this many quads)
-constexpr int kWtvLanes = 8;
+constexpr int kWtvLanes = 4;
 
 // Argument bundles of the four parts of the two products.
 // bit patterns of T: weights read through them are wave-uniform loads the
