@@ -120,6 +120,19 @@ def kernel_bytes(name, p, prec_bytes, c):
     return 0
 
 
+def leapfrog_micro(c, eps=0.01, reps=5):
+    """The fixed-step leapfrog alone (L = 25: L + 1 gradient evaluations
+    and the energies at both ends, no momentum draw, accept or Gibbs draws),
+    `reps` times after one untimed run."""
+    c.leapfrog(eps, L_STEPS)
+    t = time.perf_counter()
+    for _ in range(reps):
+        c.leapfrog(eps, L_STEPS)
+    dt = (time.perf_counter() - t) / reps
+    return {"steps": L_STEPS, "eps": eps, "ms_per_leapfrog": round(1e3 * dt, 3),
+            "grad_evals_per_s": round((L_STEPS + 1) / dt, 1)}
+
+
 def profile_kernels(c, p, iters=3):
     c.profile(True)
     for _ in range(iters):
@@ -587,6 +600,23 @@ def cpu_baseline(p, eps):
     return out
 
 
+def cpu_leapfrog(p, precision, threads, eps=0.01):
+    """One fixed-step L = 25 leapfrog of the C++ CPU restatement (no
+    momentum draw, accept or Gibbs draws) at the benchmark workload."""
+    from oracle import cpu_port
+    c = cpu_port.CpuChain(p["ijk"], THETA, p["q_hr"], EMBED, p["Yh"], p["Ys"], p["W"], precision=precision,
+                          threads=threads, w_threads=1, seed=42)
+    c.add_data_to_mu()
+    c.set_sigma_sq_inv([1.0, 4.0])
+    t = time.perf_counter()
+    c.leapfrog(eps, L_STEPS)
+    dt = time.perf_counter() - t
+    c.close()
+    log(f"[bench] cpu {precision} leapfrog alone: {dt:.1f}s")
+    return {"precision": precision, "threads": threads, "seconds": round(dt, 2),
+            "grad_evals_s": round((L_STEPS + 1) / dt, 3)}
+
+
 def cpu_study(p, eps, iters=3):
     """Medians of `iters` iterations: float32 and float64, all threads and 80%
     of them (the reference's default share, defines.h:73-74) with W on one
@@ -600,8 +630,10 @@ def cpu_study(p, eps, iters=3):
             med = float(np.median(ts))
             runs.append({"precision": prec, "threads": threads, "w_threads": w, "iteration_s": [round(x, 2) for x in ts],
                          "median_s": round(med, 2), "grad_evals_s": round((L_STEPS + 1) / med, 3)})
+    lfs = [cpu_leapfrog(p, prec, hc["threads"]) for prec in ("fp32", "fp64")]
     cfg1 = cpu_config1_chain(eps=0.1, threads=hc["threads"])
     return {"host": hc, "sample": f"median of {iters} {CPU_SAMPLE}", "eps": eps, "runs": runs,
+            "leapfrog_microbench": lfs,
             "config1_chain": dict(cfg1, sample="400 HMC iterations (L = 25, FP64, fixed step size 0.1) of the same "
                                                "restatement at the 2 mm single-resolution workload")}
 
@@ -670,6 +702,8 @@ def main():
         if prec == a.precision:
             main_sum = c.summary_read()[0].copy()
             main_chain = c
+            lf = leapfrog_micro(c)
+            log(f"[bench] {prec} leapfrog alone: {lf['grad_evals_per_s']:.1f} grad evals/s")
             ks, nk = profile_kernels(c, p)
         else:
             c.close()
@@ -738,6 +772,7 @@ def main():
                                 "p50_ms": round(rc["p50_ms"], 3), "p99_ms": round(rc["p99_ms"], 3),
                                 "bitwise_equal_to_eager": rc["bitwise_equal_to_eager"]},
         "iteration_latency_ms": {"p50": round(r["p50_ms"], 3), "p99": round(r["p99_ms"], 3)},
+        "leapfrog_microbench": lf,
         "kernel_ms_per_leapfrog_step": {k: round(v, 5) for k, v in ks.items()},
         "fft_passes_hbm_gbs": round(fft_bytes / (fft_ms * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
