@@ -505,16 +505,17 @@ struct Chain : public ChainBase {
   }
   // full-torus noise field -> spectral momentum (or initial mu)
   template <class Epi>
+  // full-torus noise field -> spectral field; the epilogue's sums go to slot
   void noise_r2c(uint64_t stream_id, const T* zinj, int slot, Epi epi) {
     if (zinj) {
-      xr2c<T>(M[1] * M[2], LoadNoiseBuf<T>{zinj, M[0], pslot(slot), 0.0}, StoreRowC<T>{{}, T2.p, H0}, slot);
+      xr2c<T>(M[1] * M[2], LoadNoiseBuf<T>{zinj, M[0], nullptr, 0.0}, StoreRowC<T>{{}, T2.p, H0});
     } else {
-      xr2c<T>(M[1] * M[2], LoadNoise<T>{cfg.seed, stream_id, M[0] / 2, pslot(slot), 0.0, gsc, (int)(stream_id & 1)},
-              StoreRowC<T>{{}, T2.p, H0}, slot);
+      xr2c<T>(M[1] * M[2], LoadNoise<T>{cfg.seed, stream_id, M[0] / 2, nullptr, 0.0, gsc, (int)(stream_id & 1)},
+              StoreRowC<T>{{}, T2.p, H0});
     }
     strided<T, false>(1, M[2], H0, M[1], M[1], LoadS<T>{{}, T2.p, (long)M[1] * H0, H0},
                       StoreS<T>{{}, T1.p, (long)M[1] * H0, H0});
-    strided<T, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0}, epi);
+    strided<T, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0}, epi, slot);
   }
 
   void finalize() {
@@ -1499,11 +1500,11 @@ struct Chain : public ChainBase {
     clear_slots();
     if (zr) upload_real(zr, zbuf);
     noise_r2c(4 * it + 0, zr ? zbuf.p : nullptr, S_E0,
-              EpiNoiseMomentum<T>{{}, p.p, invlam.p, sk, (T)sigma[0], gsc});
+              EpiNoiseMomentum<T>{p.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0), 0.0});
     if (compat()) {
       if (zr) upload_real(zi, zbuf);
       noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I,
-                EpiNoiseMomentum<T>{{}, pim.p, invlam.p, sk, (T)sigma[0], gsc});
+                EpiNoiseMomentum<T>{pim.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0I), 0.0});
     }
     leapfrog_core(eps, L);
     finalize();
@@ -1515,7 +1516,7 @@ struct Chain : public ChainBase {
   // copies enqueued on s.
   void finish_iteration(const double* h, double u, dr_update_result* res, bool copy, hipStream_t s) override {
     const double nN = (double)N;
-    const double e0 = -0.5 * (h[S_E0] + (compat() ? h[S_E0I] : 0.0));
+    const double e0 = -0.5 * (h[S_E0] + (compat() ? h[S_E0I] : 0.0)) / nN;
     const double lp0 = -0.5 * (h[S_PRIOR_A] + (compat() ? h[S_PRIORI_A] : 0.0)) / nN;
     const double ll0 = log_lik(h[S_SSEH_A], h[S_SSES_A]);
     const double lp1 = -0.5 * (h[S_PRIOR_B] + (compat() ? h[S_PRIORI_B] : 0.0)) / nN;

@@ -197,16 +197,34 @@ template <typename T> struct EpiGrad : NoRed {
 };
 
 // momentum draw p = sqrt(lambda_m) F(z), 0 on Z (MultiResParameters.h:811-814)
-template <typename T> struct EpiNoiseMomentum : NoRed {
+// p = sqrt(lambda_m) F(z), 0 on Z (MultiResParameters.h:800-815), and the
+// sum that gives E0 = -1/2 sum |z|^2 (:787-799) by Parseval: w |p|^2 /
+// lambda_m off Z, term for term as the final half kick sums the end point's
+// kinetic energy (EpiMomentum), plus w |F(z)|^2 on Z, whose energy the
+// reference's E0 holds although p does not.  Summing the spectrum rather
+// than z matters in float32: a transform rounds the field's energy by a few
+// ulp, and that difference between E0 and the end point's kinetic energy
+// would shift every accept ratio (by about exp(2.7) at the 1 mm workload).
+template <typename T> struct EpiNoiseMomentum {
   cx<T>* p; const T* invlam; long sk; T sig;
   const StepScalars<T>* sc;
+  int H0; double* part; double acc;
   __device__ __forceinline__ void bind() {
     if (sc) sig = sc->sig0;
   }
-  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) const {
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) {
     const long i = (long)k * sk + b;
     const T il = invlam[i];
-    p[i] = il > (T)0 ? v * (T)sqrt((double)(il + sig)) : mk<T>(0, 0);
+    const cx<T> pn = il > (T)0 ? v * (T)sqrt((double)(il + sig)) : mk<T>(0, 0);
+    p[i] = pn;
+    if (part) {
+      const double e = il > (T)0 ? 1.0 / (double)(il + sig) * ((double)pn.x * pn.x + (double)pn.y * pn.y)
+                                 : (double)v.x * v.x + (double)v.y * v.y;
+      acc += herm_w(b % H0, H0) * e;
+    }
+  }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (part) red_write(acc, red, part, blk);
   }
 };
 
@@ -283,7 +301,9 @@ template <typename T> struct LoadNoise {
     acc += (double)a * (double)a + (double)b * (double)b;
     return mk<T>(a, b);
   }
-  __device__ __forceinline__ void finish(double* red, int blk) { red_write(acc, red, part, blk); }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (part) red_write(acc, red, part, blk);
+  }
 };
 // injected normal field (test hook), same E0 reduction
 template <typename T> struct LoadNoiseBuf {
@@ -294,7 +314,9 @@ template <typename T> struct LoadNoiseBuf {
     acc += (double)a * (double)a + (double)b * (double)b;
     return mk<T>(a, b);
   }
-  __device__ __forceinline__ void finish(double* red, int blk) { red_write(acc, red, part, blk); }
+  __device__ __forceinline__ void finish(double* red, int blk) {
+    if (part) red_write(acc, red, part, blk);
+  }
 };
 
 // likelihood residual on the image region, scattered onto the torus rows

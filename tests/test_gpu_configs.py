@@ -251,6 +251,30 @@ def test_dual_res_1mm_fp32_chain_tracks_fp64(gpu_available, mm1):
         c.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dual_res_1mm_energy_consistent_at_zero_step(gpu_available, mm1, precision):
+    """At eps -> 0 the proposal is the start point, so H1 - H0 -> 0: each
+    energy piece of the end point must match the start's.  E0 is summed from
+    the fresh momentum's spectrum as the end point's kinetic energy is; summed
+    from the normals instead, float32 rounding of the transforms left
+    k1 - e0 = +2.7 at this size (about 2e7 modes)."""
+    p = mm1
+    c = engine_chain(p, THETA2, precision=precision, state_mode="compat", embed=EMBED_1MM, seed=8)
+    c.add_data_to_mu()
+    c.set_sigma_sq_inv([1.0, 4.0])
+    rng = np.random.default_rng(9)
+    for _ in range(2):
+        z = (rng.standard_normal(c.N), rng.standard_normal(c.N))
+        r = c.update(1e-6, 3, 2.0, noise=z)
+        print(f"1 mm {precision} eps 1e-6: k1 - e0 {r['k1'] - r['e0']:+.4f}, H1 - H0 {r['H1'] - r['H0']:+.4f}",
+              flush=True)
+        assert abs(r["k1"] - r["e0"]) < 0.1
+        assert abs(r["H1"] - r["H0"]) < 0.1
+        # E0 is -1/2 per torus point and stream, to within sampling error
+        assert abs(r["e0"] / c.N + 1.0) < 5.0 / np.sqrt(c.N)
+    c.close()
+
+
 # ---------------------------------------------------------------- other transform lengths
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

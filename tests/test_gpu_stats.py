@@ -6,9 +6,10 @@ draws them from std::mt19937, MultiResParameters.h:743-759, 783-815, so the
 values cannot match draw for draw).  These tests check what the reference's
 algorithm fixes regardless of the generator:
 
-* the fresh momentum's kinetic energy equals E0 = -1/2 sum |z|^2 (Parseval),
-  and the normals recovered from the momentum are iid N(0, 1), independent
-  between the real and imaginary streams and between iterations;
+* E0, which the engine sums from the fresh momentum as the end point's
+  kinetic energy is summed, equals -1/2 sum |z|^2 of the normals recovered
+  from that momentum (Parseval), and those normals are iid N(0, 1),
+  independent between the real and imaginary streams and between iterations;
 * the initial state is a GP draw scaled by 1/10 (its prior energy);
 * with the noise precision held fixed, the chain's marginal means and
   variances at the data voxels match the exact Gaussian posterior within
