@@ -693,20 +693,27 @@ def main():
         return
     results = {}
     main_chain = None
+    means = {}
     for prec in ((a.precision,) if a.main_only else ("fp32", "fp64")):
         c = make_chain(p, prec, "compat")
         r = run_chain(c, p, a.steps, a.warmup)
         log(f"[bench] {prec} compat: {r['grad_evals_s']:.1f} grad evals/s, {r['ms_per_iter']:.2f} ms/iter, "
             f"mh {r['mh_rate']:.2f}, eps {r['eps']:.3g}")
         results[prec] = r
+        s1, _, _, ns = c.summary_read()
+        means[prec] = s1 / max(ns, 1)
         if prec == a.precision:
-            main_sum = c.summary_read()[0].copy()
+            main_sum = s1.copy()
             main_chain = c
             lf = leapfrog_micro(c)
             log(f"[bench] {prec} leapfrog alone: {lf['grad_evals_per_s']:.1f} grad evals/s")
             ks, nk = profile_kernels(c, p)
         else:
             c.close()
+    mean_rel = None
+    if len(means) == 2:
+        mean_rel = float(np.linalg.norm(means["fp32"] - means["fp64"]) / np.linalg.norm(means["fp64"]))
+        log(f"[bench] fp32 vs fp64 posterior mean over the saved samples: relative difference {mean_rel:.2e}")
     # the same fit with every iteration replayed from a captured HIP graph
     cc = make_chain(p, a.precision, "compat")
     rc = run_chain(cc, p, a.steps, a.warmup, captured=True)
@@ -767,6 +774,9 @@ def main():
             "W_layout": wst,
         },
         "fp64_value": round(results["fp64"]["grad_evals_s"], 2) if "fp64" in results else None,
+        # same seed, same normals, uniforms and gammas: the FP32-gradient chain's
+        # posterior mean over its saved samples against the FP64 chain's
+        "fp32_vs_fp64_posterior_mean_rel": None if mean_rel is None else round(mean_rel, 8),
         "fp32_real_state_value": round(rr["grad_evals_s"], 2) if rr else None,
         "captured_iterations": {"value": round(rc["grad_evals_s"], 2), "ms_per_step": round(rc["ms_per_iter"], 4),
                                 "p50_ms": round(rc["p50_ms"], 3), "p99_ms": round(rc["p99_ms"], 3),
