@@ -7,13 +7,18 @@
 //  2. Host: rows with equal (count, hash) share one kriging system; equality
 //     is then verified element by element on the device (k_verify).
 //  3. Unique systems, sorted by size and padded with an identity block to a
-//     multiple of 64, are factorised in batches by a blocked right-looking
-//     float64 Cholesky (k_potrf / k_trsm over 64x64 tiles, the trailing
-//     update k_syrk_mfma on the FP64 matrix cores) and solved (k_solve), then
+//     multiple of 64, are factorised in batches by a blocked left-looking
+//     float64 Cholesky over 64x64 tiles, two launches per tile
+//     column: k_chol_diag subtracts the finished columns' products from the
+//     diagonal tile, factors it and inverts the factor; k_chol_panel does
+//     the same subtraction for the tiles below and multiplies them by that
+//     inverse (the products on the FP64 matrix cores).  Then solved by tiles
+//     with those inverses (k_solve_chol), then
 //     normalised by their sum (kriging_matrix.h:318-321).  Systems whose
 //     Cholesky meets a nonpositive pivot (K with its unit diagonal is
 //     indefinite when tau^2 > 1) are factorised again by a blocked LU with
-//     partial pivoting (k_getrf_panel / k_trsm_lu / k_gemm_lu_mfma).
+//     partial pivoting (k_getrf_panel / k_trsm_lu / k_gemm_lu_mfma) and
+//     solved by k_solve_lu.
 //  4. Rows gather their system's weights; rows whose weights sum to NaN stay
 //     empty, as in the reference.
 // K and k' entries are float32 (the reference evaluates rbf<float>), taken as
@@ -184,16 +189,18 @@ struct SysDesc {
   long sol;   // offset into the solution buffer
 };
 
-// A_s = [K[pidx, pidx] 0; 0 I] padded to np; b_s = [k'; 0]
+// A_s = [K[pidx, pidx] 0; 0 I] padded to np; b_s = [k'; 0].  lower != 0
+// skips the 64x64 tiles above the diagonal, which the Cholesky never reads.
 __global__ void k_assemble(const SysDesc* __restrict__ sd, int np, const float* __restrict__ K, int nP,
                            const int* __restrict__ pidx, const float* __restrict__ kp, double* __restrict__ A,
-                           double* __restrict__ bv) {
+                           double* __restrict__ bv, int lower) {
   const int s = blockIdx.y;
   const SysDesc d = sd[s];
   const long npp = (long)np * np;
   double* As = A + (long)s * npp;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npp; t += (long)gridDim.x * blockDim.x) {
     const int i = (int)(t / np), j = (int)(t - (long)i * np);
+    if (lower && j / NB > i / NB) continue;
     double v;
     if (i < d.n && j < d.n) v = (double)K[(long)pidx[d.off + i] * nP + pidx[d.off + j]];
     else v = (i == j) ? 1.0 : 0.0;
@@ -203,128 +210,225 @@ __global__ void k_assemble(const SysDesc* __restrict__ sd, int np, const float* 
     for (int i = threadIdx.x; i < np; i += blockDim.x) bv[(long)s * np + i] = i < d.n ? (double)kp[d.off + i] : 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_potrf(double* A, int np, int k, int* fail) {
-  __shared__ double L[NB][NB + 1];
-  double* As = A + (long)blockIdx.x * np * np;
-  const int o = k * NB;
-  for (int t = threadIdx.x; t < NB * NB; t += 256) {
-    const int i = t / NB, j = t % NB;
-    L[i][j] = As[(long)(o + i) * np + o + j];
-  }
-  __syncthreads();
-  for (int j = 0; j < NB; ++j) {
-    const double d = L[j][j];
-    __syncthreads();
-    const double sd = sqrt(d);
-    if (threadIdx.x == 0) {
-      L[j][j] = sd;
-      if (!(d > 0.0)) fail[blockIdx.x] = 1;
-    }
-    for (int i = j + 1 + threadIdx.x; i < NB; i += 256) L[i][j] /= sd;
-    __syncthreads();
-    const int m = NB - j - 1;
-    for (int t = threadIdx.x; t < m * m; t += 256) {
-      const int ii = j + 1 + t / m, ll = j + 1 + t % m;
-      if (ll <= ii) L[ii][ll] -= L[ii][j] * L[ll][j];
-    }
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < NB * NB; t += 256) {
-    const int i = t / NB, j = t % NB;
-    if (j <= i) As[(long)(o + i) * np + o + j] = L[i][j];
-  }
-}
-
-__global__ void __launch_bounds__(256) k_trsm(double* A, int np, int k) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ double X[NB][NB + 1];
-  double* As = A + (long)blockIdx.x * np * np;
-  const int o = k * NB;
-  const int r0 = (k + 1 + blockIdx.y) * NB;
-  for (int t = threadIdx.x; t < NB * NB; t += 256) {
-    const int i = t / NB, j = t % NB;
-    L[i][j] = As[(long)(o + i) * np + o + j];
-    X[i][j] = As[(long)(r0 + i) * np + o + j];
-  }
-  __syncthreads();
-  for (int j = 0; j < NB; ++j) {
-    const double d = L[j][j];
-    for (int i = threadIdx.x; i < NB; i += 256) X[i][j] /= d;
-    __syncthreads();
-    const int m = NB - j - 1;
-    for (int t = threadIdx.x; t < NB * m; t += 256) {
-      const int i = t / m, l = j + 1 + t % m;
-      X[i][l] -= X[i][j] * L[l][j];
-    }
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < NB * NB; t += 256) {
-    const int i = t / NB, j = t % NB;
-    As[(long)(r0 + i) * np + o + j] = X[i][j];
-  }
-}
-
-// Trailing updates on the FP64 matrix cores: A[BI][BJ] -= P[BI] Q, where P
-// is the 64-column panel of the tile row BI and Q the 64 x 64 block that
-// multiplies it: P[BJ]^T for the Cholesky (k_syrk_mfma) and the row block of U
-// above BJ for the LU (k_gemm_lu_mfma).  Each wavefront owns a 32x32 quadrant
-// of the 64x64 tile as 2x2 blocks of 16x16, accumulated by
-// v_mfma_f64_16x16x4f64 over the 64 panel columns, 4 at a time.  Operand
-// layout of the instruction (checked on the device with distinct per-lane
-// values): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
-// accumulator register j of lane l holds D[4 j + l / 16][l % 16].
+// FP64 matrix-core tile products.  Each wavefront owns a 32x32 quadrant of a
+// 64x64 tile as 2x2 blocks of 16x16, accumulated by v_mfma_f64_16x16x4f64
+// over 64 columns, 4 at a time.  Operand layout of the instruction (checked
+// on the device with distinct per-lane values): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register t of lane l
+// holds D[4 t + l / 16][l % 16].
 typedef double dr_v4f64 __attribute__((ext_vector_type(4)));
-template <bool LU>
-__device__ __forceinline__ void tile_update_mfma(double* As, int np, int BI, int BJ, int o) {
-  __shared__ double Pi[NB][NB + 1];
-  __shared__ double Pj[NB][NB + 1];  // Pj[col][k]: Q^T
-  for (int q = threadIdx.x; q < NB * NB; q += 256) {
-    const int i = q / NB, l = q % NB;
-    Pi[i][l] = As[(long)(BI + i) * np + o + l];
-    if (LU) Pj[l][i] = As[(long)(o + i) * np + BJ + l];  // U[o + i][BJ + l]
-    else Pj[i][l] = As[(long)(BJ + i) * np + o + l];
+typedef double Tile[NB][NB + 1];
+
+struct Quad {  // this thread's place in the tile products
+  int r0, c0, li, lk;
+  __device__ Quad() {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    r0 = (wave >> 1) * 32;
+    c0 = (wave & 1) * 32;
+    li = lane & 15;
+    lk = lane >> 4;
   }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
-  const int li = lane & 15, lk = lane >> 4;
-  dr_v4f64 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = dr_v4f64{0.0, 0.0, 0.0, 0.0};
+  // tile row / column of accumulator register t of block (a, b)
+  __device__ int row(int a, int t) const { return r0 + 16 * a + 4 * t + lk; }
+  __device__ int col(int b) const { return c0 + 16 * b + li; }
+};
+
+// acc += X Y^T over 64 columns, X and Y row-major 64x64 tiles in LDS
+__device__ __forceinline__ void mfma_xyt(const Tile& X, const Tile& Y, const Quad& q, dr_v4f64 acc[2][2]) {
 #pragma unroll 4
   for (int kk = 0; kk < NB; kk += 4) {
     double av[2], bv[2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = Pi[r0 + 16 * a + li][kk + lk];
+    for (int a = 0; a < 2; ++a) av[a] = X[q.r0 + 16 * a + q.li][kk + q.lk];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) bv[b] = Pj[c0 + 16 * b + li][kk + lk];
+    for (int b = 0; b < 2; ++b) bv[b] = Y[q.c0 + 16 * b + q.li][kk + q.lk];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void zero_acc(dr_v4f64 acc[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = dr_v4f64{0.0, 0.0, 0.0, 0.0};
+}
+
+// Left-looking Cholesky, tile (BI, BJ) of tile column j:
+// cv = A[BI][BJ] - sum_{l<j} L[BI][l] L[BJ][l]^T, accumulated over the whole
+// depth j * 64 in registers (a right-looking update reads and writes every
+// trailing tile once per panel).  The tile pair of panel l + 1 is fetched
+// into registers while the matrix cores work on panel l from LDS.  Returns
+// with every thread past its last read of Pi and Pj.
+__device__ __forceinline__ void col_update(const double* As, int np, int BI, int BJ, int j, Tile& Pi, Tile& Pj,
+                                           const Quad& q, double cv[2][2][4]) {
+  const int lc = threadIdx.x & 63, lr = threadIdx.x >> 6;  // loads: column lc of rows lr + 4 r
+  double gi[16], gj[16];
+  auto fetch = [&](int l) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      gi[r] = As[(long)(BI + lr + 4 * r) * np + l * NB + lc];
+      gj[r] = As[(long)(BJ + lr + 4 * r) * np + l * NB + lc];
+    }
+  };
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) cv[a][b][t] = As[(long)(BI + q.row(a, t)) * np + BJ + q.col(b)];
+  dr_v4f64 acc[2][2];
+  zero_acc(acc);
+  if (j > 0) fetch(0);
+  for (int l = 0; l < j; ++l) {
+    __syncthreads();  // the previous panel's products are done with LDS
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      Pi[lr + 4 * r][lc] = gi[r];
+      Pj[lr + 4 * r][lc] = gj[r];
+    }
+    __syncthreads();
+    if (l + 1 < j) fetch(l + 1);
+    mfma_xyt(Pi, Pj, q, acc);
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = r0 + 16 * a + 4 * j + lk, col = c0 + 16 * b + li;
-        As[(long)(BI + row) * np + BJ + col] -= acc[a][b][j];
-      }
+      for (int t = 0; t < 4; ++t) cv[a][b][t] -= acc[a][b][t];
+  __syncthreads();
 }
 
-// Cholesky: lower tiles (bi >= bj) of the trailing matrix, blockIdx.y = the
-// tile's index in the packed lower triangle
-__global__ void __launch_bounds__(256) k_syrk_mfma(double* A, int np, int k) {
-  const int t = blockIdx.y;
-  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
-  while (bi * (bi + 1) / 2 > t) --bi;
-  const int bj = t - bi * (bi + 1) / 2;
-  tile_update_mfma<false>(A + (long)blockIdx.x * np * np, np, (k + 1 + bi) * NB, (k + 1 + bj) * NB, k * NB);
+__device__ __forceinline__ void cv_to_lds(const double cv[2][2][4], const Quad& q, Tile& T) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) T[q.row(a, t)][q.col(b)] = cv[a][b][t];
+}
+
+// value of lane l (uniform) of a double held across the wavefront
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One wavefront factors the 64x64 tile T in place, lane c holding row c in
+// registers: for j = 0..63, L[j][j] = sqrt(A[j][j]), L[c][j] = A[c][j] /
+// L[j][j] for c > j, then A[c][m] -= L[c][j] L[m][j] for m > j (rows c < m
+// update entries above the diagonal, which are never read).  Returns 1 on a
+// nonpositive pivot.
+__device__ __forceinline__ int potrf_wave(Tile& T) {
+  const int c = threadIdx.x & 63;
+  double a[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) a[m] = T[c][m];
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const double d = lane_value(a[j], j);
+    bad |= !(d > 0.0);
+    const double sd = sqrt(d);
+    a[j] = c == j ? sd : a[j] / sd;
+#pragma unroll
+    for (int m = j + 1; m < NB; ++m) a[m] -= a[j] * lane_value(a[j], m);
+  }
+#pragma unroll
+  for (int m = 0; m < NB; ++m) T[c][m] = a[m];
+  return bad;
+}
+
+// The inverse of the lower factor in T, by the whole workgroup: diagonal
+// blocks of size s = 1, 2, 4, .. 32 are inverted from those of size s / 2 by
+// [A 0; B C]^-1 = [A^-1 0; -C^-1 B A^-1  C^-1], the product B A^-1 kept for
+// the second product in the block's unused upper triangle of W.  Leaves
+// L^-1[i][c] at T[c][i] for i > c (the inverse's diagonal is 1 / L[i][i]);
+// W is scratch.
+__device__ __forceinline__ void trtri_block(Tile& T, Tile& W) {
+  for (int t = threadIdx.x; t < NB; t += 256) W[t][t] = 1.0 / T[t][t];
+  __syncthreads();
+  for (int s = 1; s < NB; s *= 2) {
+    const int n = (NB / (2 * s)) * s * s;  // elements of all blocks at this level
+    for (int t = threadIdx.x; t < n; t += 256) {  // U[cc][r] = (B A^-1)[r][cc]
+      const int b = (t / (s * s)) * 2 * s, r = (t / s) % s, cc = t % s;
+      double v = 0.0;
+      for (int k = cc; k < s; ++k) v += T[b + s + r][b + k] * W[b + k][b + cc];
+      W[b + cc][b + s + r] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += 256) {  // X[r][cc] = -(C^-1 U^T)[r][cc]
+      const int b = (t / (s * s)) * 2 * s, r = (t / s) % s, cc = t % s;
+      double v = 0.0;
+      for (int k = 0; k <= r; ++k) v += W[b + s + r][b + s + k] * W[b + cc][b + s + k];
+      W[b + s + r][b + cc] = -v;
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, c = t % NB;
+    if (i > c) T[c][i] = W[i][c];
+  }
+}
+
+__device__ __forceinline__ void lds_to_tile(const Tile& T, double* As, int np, int BI, int BJ) {
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, jj = t % NB;
+    As[(long)(BI + i) * np + BJ + jj] = T[i][jj];
+  }
+}
+
+// Cholesky, tile column j, diagonal tile: update, factor, invert
+__global__ void __launch_bounds__(256) k_chol_diag(double* A, int np, int j, int* fail) {
+  __shared__ Tile Pi, Pj;
+  double* As = A + (long)blockIdx.x * np * np;
+  const Quad q;
+  double cv[2][2][4];
+  col_update(As, np, j * NB, j * NB, j, Pi, Pj, q, cv);
+  cv_to_lds(cv, q, Pi);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int bad = potrf_wave(Pi);
+    if (bad && threadIdx.x == 0) fail[blockIdx.x] = 1;
+  }
+  __syncthreads();
+  trtri_block(Pi, Pj);
+  __syncthreads();
+  lds_to_tile(Pi, As, np, j * NB, j * NB);
+}
+
+// Cholesky, tile column j, the tile rows i = j + 1 + blockIdx.y below the
+// diagonal: update, then L[i][j] = C L[j][j]^-T as a product with the
+// inverse k_chol_diag left above the diagonal (in place of a
+// column-by-column substitution, 64 dependent steps per tile)
+__global__ void __launch_bounds__(256) k_chol_panel(double* A, int np, int j) {
+  __shared__ Tile Pi, Pj;
+  double* As = A + (long)blockIdx.x * np * np;
+  const int BI = (j + 1 + blockIdx.y) * NB, o = j * NB;
+  const Quad q;
+  double cv[2][2][4];
+  col_update(As, np, BI, o, j, Pi, Pj, q, cv);
+  cv_to_lds(cv, q, Pi);
+  const double* D = As + (long)o * np + o;
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {  // Pj[i][l] = L^-1[i][l]
+    const int i = t / NB, l = t % NB;
+    Pj[i][l] = l < i ? D[(long)l * np + i] : l == i ? 1.0 / D[(long)i * np + i] : 0.0;
+  }
+  __syncthreads();
+  dr_v4f64 acc[2][2];
+  zero_acc(acc);
+  mfma_xyt(Pi, Pj, q, acc);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) As[(long)(BI + q.row(a, t)) * np + o + q.col(b)] = acc[a][b][t];
 }
 
 // ---------------------------------------------------------------- LU with partial pivoting
@@ -435,9 +539,27 @@ __global__ void __launch_bounds__(256) k_trsm_lu(double* A, int np, int k) {
 }
 
 // A22 -= L21 U12 over every trailing tile, blockIdx.y = bi * m + bj
+// on the FP64 matrix cores
 __global__ void __launch_bounds__(256) k_gemm_lu_mfma(double* A, int np, int k, int m) {
-  const int bi = blockIdx.y / m, bj = blockIdx.y % m;
-  tile_update_mfma<true>(A + (long)blockIdx.x * np * np, np, (k + 1 + bi) * NB, (k + 1 + bj) * NB, k * NB);
+  __shared__ Tile Pi, Pj;
+  double* As = A + (long)blockIdx.x * np * np;
+  const int BI = (k + 1 + blockIdx.y / m) * NB, BJ = (k + 1 + blockIdx.y % m) * NB, o = k * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += 256) {
+    const int i = t / NB, l = t % NB;
+    Pi[i][l] = As[(long)(BI + i) * np + o + l];  // L21
+    Pj[l][i] = As[(long)(o + i) * np + BJ + l];  // U12^T
+  }
+  __syncthreads();
+  const Quad q;
+  dr_v4f64 acc[2][2];
+  zero_acc(acc);
+  mfma_xyt(Pi, Pj, q, acc);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) As[(long)(BI + q.row(a, t)) * np + BJ + q.col(b)] -= acc[a][b][t];
 }
 
 __device__ __forceinline__ double wsum(double v) {
@@ -447,10 +569,8 @@ __device__ __forceinline__ double wsum(double v) {
 }
 
 // Solves the factored system and normalises: w /= sum(w) (a zero sum counts
-// as 1, kriging_matrix.h:318-321); writes the n weights to sol.  LU = false:
-// L L^T w = b with L the Cholesky factor (lower triangle of A).  LU = true:
-// L U w = b with b already permuted, L unit lower and U upper in A; a system
-// flagged singular gets NaN weights, which drop its rows as a NaN sum does.
+// as 1, kriging_matrix.h:318-321); writes the n weights to sol.  One
+// workgroup per system; y holds the right-hand side, then the solution.
 __device__ __forceinline__ double block_dot(double v, double* part) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   v = wsum(v);
@@ -461,65 +581,156 @@ __device__ __forceinline__ double block_dot(double v, double* part) {
   return t;
 }
 
-template <bool LU>
-__global__ void __launch_bounds__(256) k_solve(const double* A, int np, const SysDesc* __restrict__ sd,
-                                               const double* __restrict__ bv, const int* __restrict__ fail,
-                                               double* __restrict__ sol) {
+__device__ __forceinline__ void normalise_out(const double* y, int n, double* part, double* out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += y[i];
+  double tot = block_dot(acc, part);
+  if (tot == 0.0) tot = 1.0;
+  for (int i = threadIdx.x; i < n; i += 256) out[i] = y[i] / tot;
+}
+
+// Cholesky: L L^T w = b by 64-row tiles, with the inverses of the diagonal
+// tiles that k_chol_diag left above their diagonals.  Forward, tile k:
+// r = b_k - sum_{l<k} L[k][l] y_l, y_k = L[k][k]^-1 r; backward:
+// r = y_k - sum_{l>k} L[l][k]^T x_l, x_k = L[k][k]^-T r.  Tiles past the
+// system's size are identity padding with a zero right-hand side and are
+// skipped.
+__global__ void __launch_bounds__(256) k_solve_chol(const double* A, int np, const SysDesc* __restrict__ sd,
+                                                    const double* __restrict__ bv, double* __restrict__ sol) {
+  extern __shared__ double y[];
+  __shared__ double part[4][NB];
+  __shared__ double rk[NB];
+  const int s = blockIdx.x;
+  const SysDesc d = sd[s];
+  const double* As = A + (long)s * np * np;
+  const int nt = (d.n + NB - 1) / NB, nn = nt * NB;
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < nn; i += 256) y[i] = bv[(long)s * np + i];
+  __syncthreads();
+  for (int k = 0; k < nt; ++k) {
+    const int o = k * NB;
+    const double* D = As + (long)o * np + o;
+    double acc[16];  // rows o + 16 q + rr, the lanes over the columns
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) acc[rr] = 0.0;
+    for (int c = lane; c < o; c += 64) {
+      const double yc = y[c];
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) acc[rr] += As[(long)(o + 16 * q + rr) * np + c] * yc;
+    }
+    double mine = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+      const double v = wsum(acc[rr]);
+      if (lane == rr) mine = v;
+    }
+    if (lane < 16) rk[16 * q + lane] = y[o + 16 * q + lane] - mine;
+    __syncthreads();
+    double v = 0.0;  // (L^-1 r)[lane], columns 16 q ..
+#pragma unroll 4
+    for (int c = 16 * q; c < 16 * q + 16; ++c) {
+      const double lic = c < lane ? D[(long)c * np + lane] : c == lane ? 1.0 / D[(long)lane * np + lane] : 0.0;
+      v += lic * rk[c];
+    }
+    part[q][lane] = v;
+    __syncthreads();
+    if (q == 0) y[o + lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    __syncthreads();
+  }
+  for (int k = nt - 1; k >= 0; --k) {
+    const int o = k * NB;
+    const double* D = As + (long)o * np + o;
+    double v = 0.0;  // column o + lane, rows o + 64 + q, step 4
+    for (int i = o + NB + q; i < nn; i += 4) v += As[(long)i * np + o + lane] * y[i];
+    part[q][lane] = v;
+    __syncthreads();
+    if (q == 0) rk[lane] = y[o + lane] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+    __syncthreads();
+    v = 0.0;  // (L^-T r)[lane], rows 16 q ..
+#pragma unroll 4
+    for (int i = 16 * q; i < 16 * q + 16; ++i) {
+      const double lic = i > lane ? D[(long)lane * np + i] : i == lane ? 1.0 / D[(long)lane * np + lane] : 0.0;
+      v += lic * rk[i];
+    }
+    part[q][lane] = v;
+    __syncthreads();
+    if (q == 0) y[o + lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    __syncthreads();
+  }
+  normalise_out(y, d.n, &part[0][0], sol + d.sol);
+}
+
+// LU: L U w = b with b already permuted, L unit lower and U upper in A; a
+// system flagged singular gets NaN weights, which drop its rows as a NaN sum
+// does.
+__global__ void __launch_bounds__(256) k_solve_lu(const double* A, int np, const SysDesc* __restrict__ sd,
+                                                  const double* __restrict__ bv, const int* __restrict__ fail,
+                                                  double* __restrict__ sol) {
   extern __shared__ double y[];
   __shared__ double part[4];
   const int s = blockIdx.x;
   const SysDesc d = sd[s];
   const double* As = A + (long)s * np * np;
   const int n = d.n;
-  if (LU && fail[s]) {
+  if (fail[s]) {
     for (int i = threadIdx.x; i < n; i += 256) sol[d.sol + i] = __builtin_nan("");
     return;
   }
   for (int i = threadIdx.x; i < n; i += 256) y[i] = bv[(long)s * np + i];
   __syncthreads();
-  // forward: row j of L against the solved y[0..j)
-  for (int j = 0; j < n; ++j) {
+  for (int j = 0; j < n; ++j) {  // forward: row j of L against the solved y[0..j)
     const double* Lj = As + (long)j * np;
     double acc = 0.0;
     for (int i = threadIdx.x; i < j; i += 256) acc += Lj[i] * y[i];
     acc = block_dot(acc, part);
-    if (threadIdx.x == 0) y[j] = LU ? y[j] - acc : (y[j] - acc) / Lj[j];
+    if (threadIdx.x == 0) y[j] -= acc;
     __syncthreads();
   }
-  if (LU) {  // back: row j of U against the solved y(j..n)
-    for (int j = n - 1; j >= 0; --j) {
-      const double* Uj = As + (long)j * np;
-      double acc = 0.0;
-      for (int i = j + 1 + (int)threadIdx.x; i < n; i += 256) acc += Uj[i] * y[i];
-      acc = block_dot(acc, part);
-      if (threadIdx.x == 0) y[j] = (y[j] - acc) / Uj[j];
-      __syncthreads();
-    }
-  } else {  // back: L^T by columns of L
-    for (int j = n - 1; j >= 0; --j) {
-      const double* Lj = As + (long)j * np;
-      const double wj = y[j] / Lj[j];
-      __syncthreads();
-      if (threadIdx.x == 0) y[j] = wj;
-      for (int i = threadIdx.x; i < j; i += 256) y[i] -= Lj[i] * wj;
-      __syncthreads();
-    }
+  for (int j = n - 1; j >= 0; --j) {  // back: row j of U against the solved y(j..n)
+    const double* Uj = As + (long)j * np;
+    double acc = 0.0;
+    for (int i = j + 1 + (int)threadIdx.x; i < n; i += 256) acc += Uj[i] * y[i];
+    acc = block_dot(acc, part);
+    if (threadIdx.x == 0) y[j] = (y[j] - acc) / Uj[j];
+    __syncthreads();
   }
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += y[i];
-  double tot = block_dot(acc, part);
-  if (tot == 0.0) tot = 1.0;
-  for (int i = threadIdx.x; i < n; i += 256) sol[d.sol + i] = y[i] / tot;
+  normalise_out(y, n, part, sol + d.sol);
 }
 
-// vals[e] = sol[sys_sol[uid(row)] + (e - rp[row])]
+// vals[e] = sol[sys_sol[uid(row)] + (e - rp[row])]; nanrow[row] = 1 when a
+// weight of the row is NaN
 __global__ void k_gather_rows(int n_rows, const long* __restrict__ rp, const int* __restrict__ uid,
-                              const long* __restrict__ usol, const double* __restrict__ sol, double* __restrict__ vals) {
+                              const long* __restrict__ usol, const double* __restrict__ sol, double* __restrict__ vals,
+                              int* __restrict__ nanrow) {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n_rows) return;
-  const long a = rp[row], n = rp[row + 1] - a, so = usol[uid[row]];
-  for (long t = lane; t < n; t += 64) vals[a + t] = sol[so + t];
+  const long a = rp[row], n = rp[row + 1] - a;
+  int nan = 0;
+  if (n > 0) {
+    const long so = usol[uid[row]];
+    for (long t = lane; t < n; t += 64) {
+      const double v = sol[so + t];
+      vals[a + t] = v;
+      nan |= v != v;
+    }
+  }
+  const uint64_t bal = __ballot(nan);
+  if (lane == 0) nanrow[row] = bal != 0;
+}
+
+// kept rows moved to their compacted offsets rpn (dropped rows have none)
+__global__ void k_compact_rows(int n_rows, const long* __restrict__ rp, const long* __restrict__ rpn,
+                               const int* __restrict__ cols, const double* __restrict__ vals, int* __restrict__ cols_out,
+                               double* __restrict__ vals_out) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const long a = rp[row], b = rpn[row], n = rpn[row + 1] - b;
+  for (long t = lane; t < n; t += 64) {
+    cols_out[b + t] = cols[a + t];
+    vals_out[b + t] = vals[a + t];
+  }
 }
 
 template <typename T>
@@ -528,6 +739,16 @@ struct DBuf {
   DBuf() = default;
   explicit DBuf(size_t n) { alloc(n); }
   DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
   ~DBuf() { if (p) (void)hipFree(p); }
   void alloc(size_t n) {
     if (p) (void)hipFree(p);
@@ -550,12 +771,13 @@ float fsq(float a) {
 }  // namespace
 
 // ---------------------------------------------------------------- host build
+// The CSR stays on the device where it was built until it is copied out.
 struct KrigingCSR {
-  long nrow = 0, ncol = 0, n_unique = 0;
+  long nrow = 0, ncol = 0, n_unique = 0, nnz = 0;
   long n_lu = 0;  // unique systems solved by the pivoted LU (Cholesky failed)
   std::vector<int64_t> rp;
-  std::vector<int32_t> cols;
-  std::vector<double> vals;
+  DBuf<int32_t> cols;
+  DBuf<double> vals;
 };
 
 static float norm3f(const float* v) {
@@ -797,7 +1019,7 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
       DR_HIP(hipMemsetAsync(d_fail.p, 0, B * sizeof(int), s));
       const int gx = std::max(1, std::min(cdiv((long)np * np, 256), 256));
       hipLaunchKernelGGL(k_assemble, dim3(gx, B), dim3(256), 0, s, d_sd.p, np, d_K.p, nP, d_pidx.p, d_kp.p, d_A.p,
-                         d_b.p);
+                         d_b.p, lu ? 0 : 1);
       DR_HIP(hipGetLastError());
       const int nt = np / NB;
       for (int k = 0; k < nt; ++k) {
@@ -809,20 +1031,17 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
             hipLaunchKernelGGL(k_gemm_lu_mfma, dim3(B, m * m), dim3(256), 0, s, d_A.p, np, k, m);
           }
         } else {
-          hipLaunchKernelGGL(k_potrf, dim3(B), dim3(256), 0, s, d_A.p, np, k, d_fail.p);
-          if (m > 0) {
-            hipLaunchKernelGGL(k_trsm, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
-            hipLaunchKernelGGL(k_syrk_mfma, dim3(B, m * (m + 1) / 2), dim3(256), 0, s, d_A.p, np, k);
-          }
+          hipLaunchKernelGGL(k_chol_diag, dim3(B), dim3(256), 0, s, d_A.p, np, k, d_fail.p);
+          if (m > 0) hipLaunchKernelGGL(k_chol_panel, dim3(B, m), dim3(256), 0, s, d_A.p, np, k);
         }
         DR_HIP(hipGetLastError());
       }
       if (lu)
-        hipLaunchKernelGGL(k_solve<true>, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
+        hipLaunchKernelGGL(k_solve_lu, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
                            d_fail.p, d_sol.p);
       else
-        hipLaunchKernelGGL(k_solve<false>, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
-                           d_fail.p, d_sol.p);
+        hipLaunchKernelGGL(k_solve_chol, dim3(B), dim3(256), np * sizeof(double), s, d_A.p, np, d_sd.p, d_b.p,
+                           d_sol.p);
       DR_HIP(hipGetLastError());
       std::vector<int> fail(B);
       d_fail.down(fail.data(), B, s);
@@ -841,34 +1060,40 @@ KrigingCSR kriging_build(const double* hr, const int hd[3], const float qh[16], 
   DBuf<long> d_usol(std::max(nU, 1));
   if (nU) d_usol.up(usol.data(), nU, s);
   DBuf<double> d_vals(nnz);
+  DBuf<int> d_nan(n_rows);
   hipLaunchKernelGGL(k_gather_rows, dim3(gblk), dim3(256), 0, s, n_rows, d_rp.p, d_uid.p, d_usol.p, d_sol.p,
-                     d_vals.p);
+                     d_vals.p, d_nan.p);
   DR_HIP(hipGetLastError());
-  KrigingCSR out;
-  std::vector<int32_t> cols(nnz);
-  std::vector<double> vals(nnz);
-  d_cols.down(cols.data(), nnz, s);
-  d_vals.down(vals.data(), nnz, s);
+  std::vector<int> nanrow(n_rows);
+  d_nan.down(nanrow.data(), n_rows, s);
   DR_HIP(hipStreamSynchronize(s));
-  // drop rows whose weight sum is NaN (kriging_matrix.h:320-327)
+  // drop rows whose weight sum is NaN (kriging_matrix.h:320-327): one NaN
+  // weight makes the sum NaN
+  KrigingCSR out;
   out.nrow = n_rows;
   out.ncol = nnz_hr;
   out.n_unique = nU;
   out.n_lu = (long)indefinite.size();
   out.rp.assign(n_rows + 1, 0);
-  out.cols.reserve(nnz);
-  out.vals.reserve(nnz);
+  long dropped = 0;
   for (int i = 0; i < n_rows; ++i) {
-    bool keep = cnt[i] > 0;
-    if (keep) {
-      for (long e = rp[i]; e < rp[i + 1]; ++e)
-        if (std::isnan(vals[e])) { keep = false; break; }
-    }
-    if (keep) {
-      out.cols.insert(out.cols.end(), cols.begin() + rp[i], cols.begin() + rp[i + 1]);
-      out.vals.insert(out.vals.end(), vals.begin() + rp[i], vals.begin() + rp[i + 1]);
-    }
-    out.rp[i + 1] = (int64_t)out.cols.size();
+    const bool drop = cnt[i] > 0 && nanrow[i];
+    dropped += drop ? 1 : 0;
+    out.rp[i + 1] = out.rp[i] + (drop ? 0 : cnt[i]);
+  }
+  out.nnz = out.rp[n_rows];
+  if (dropped == 0) {
+    out.cols = std::move(d_cols);
+    out.vals = std::move(d_vals);
+  } else {
+    DBuf<long> d_rpn(n_rows + 1);
+    d_rpn.up(out.rp.data(), n_rows + 1, s);
+    out.cols.alloc(out.nnz);
+    out.vals.alloc(out.nnz);
+    hipLaunchKernelGGL(k_compact_rows, dim3(gblk), dim3(256), 0, s, n_rows, d_rp.p, d_rpn.p, d_cols.p, d_vals.p,
+                       out.cols.p, out.vals.p);
+    DR_HIP(hipGetLastError());
+    DR_HIP(hipStreamSynchronize(s));
   }
   (void)oobv;
   return out;
@@ -897,7 +1122,7 @@ int dr_csr_info(dr_csr* m, int64_t* nrow, int64_t* ncol, int64_t* nnz, int64_t* 
     DR_REQUIRE(m, DR_ELOGIC, "null csr");
     *nrow = m->m.nrow;
     *ncol = m->m.ncol;
-    *nnz = (int64_t)m->m.cols.size();
+    *nnz = (int64_t)m->m.nnz;
     if (n_unique) *n_unique = m->m.n_unique;
   });
 }
@@ -913,9 +1138,13 @@ int dr_csr_solver_counts(dr_csr* m, int64_t* n_cholesky, int64_t* n_lu) {
 int dr_csr_copy(dr_csr* m, int64_t* rowptr, int32_t* cols, double* vals) {
   return drh::guarded([&] {
     DR_REQUIRE(m, DR_ELOGIC, "null csr");
+    DR_REQUIRE(rowptr, DR_ELOGIC, "dr_csr_copy: null argument");
     std::copy(m->m.rp.begin(), m->m.rp.end(), rowptr);
-    std::copy(m->m.cols.begin(), m->m.cols.end(), cols);
-    std::copy(m->m.vals.begin(), m->m.vals.end(), vals);
+    if (m->m.nnz > 0) {
+      DR_REQUIRE(cols && vals, DR_ELOGIC, "dr_csr_copy: null argument");
+      DR_HIP(hipMemcpy(cols, m->m.cols.p, m->m.nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+      DR_HIP(hipMemcpy(vals, m->m.vals.p, m->m.nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
   });
 }
 

@@ -252,7 +252,10 @@ typedef struct dr_csr dr_csr;
 /* hr/std: dense float64 volumes (nx*ny*nz, x fastest), 0 or NaN = missing,
  * already masked.  theta as in dr_config.  Pattern is the reference's
  * (neighbour order, flat-offset range check); weights are float64 solves
- * normalised by their sum (rows with a NaN sum are left empty). */
+ * normalised by their sum (rows with a NaN sum are left empty).  The CSR
+ * is held in the device's memory until dr_csr_free; dr_csr_copy copies it
+ * out (rowptr: nrow + 1 entries; cols and vals: nnz, may be null when
+ * nnz == 0). */
 int dr_kriging_build(const double* hr, const int hr_dims[3], const float q_hr[16],
                      const double* std_img, const int std_dims[3], const float q_std[16],
                      const double theta[3], double radius, int device, dr_csr** out);

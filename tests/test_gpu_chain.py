@@ -545,6 +545,30 @@ def test_kriging_indefinite_systems_tau2_above_one(gpu_available, dims, vox):
     assert np.allclose(np.asarray(W.sum(axis=1)).ravel()[np.diff(rp) > 0], 1.0, atol=1e-12)
 
 
+def test_kriging_singular_systems_drop_their_rows(gpu_available):
+    """psi = 0 with tau^2 = 1 makes every K (unit diagonal) a matrix of ones:
+    systems of two or more neighbours are singular, the Cholesky and then
+    the pivoted LU flag them, and their rows come out empty, as rows whose
+    weights sum to NaN do (kriging_matrix.h:320-327); rows with a single
+    neighbour keep it with weight 1, at the compacted offsets.  At radius
+    2.1 mm on this grid: 1 empty, 11 single-neighbour and 70 singular rows."""
+    from dualres_amd import problem
+    from dualres_amd.kriging import kriging_build
+    from oracle import kriging
+    p = problem.dual_problem((14, 12, 10), 2.0, (0.806, 0.132, 1.5), seed_gp=9)
+    theta, radius = (1.0, 0.0, 1.5), 2.1
+    rows, P, st = kriging.kriging_pattern(p["hr"], p["std"], p["q_hr"], p["q_std"], theta, radius)
+    cnt = np.array([r[0].size for r in rows])
+    assert (cnt == 0).sum() == 1 and (cnt == 1).sum() == 11 and (cnt >= 2).sum() == 70
+    W, info = kriging_build(p["hr"], p["q_hr"], p["std"], p["q_std"], theta, radius, return_info=True)
+    assert W.shape[0] == len(rows) and info["n_lu"] > 0
+    got = np.diff(W.indptr)
+    assert np.array_equal(got, np.where(cnt == 1, 1, 0))
+    single = np.flatnonzero(cnt == 1)
+    assert np.array_equal(W.indices, np.concatenate([rows[r][0] for r in single]))
+    assert np.array_equal(W.data, np.ones(single.size))
+
+
 def test_native_fit_loop_on_a_prepared_chain(gpu_available):
     """The C++ host's fit loop (dualgpmf's fit_model iterations) driven on a
     chain prepared through the C ABI: the same seed reproduces the run bit
