@@ -1079,6 +1079,38 @@ struct Chain : public ChainBase {
       tfw.swap(tfw2);
       tfp.swap(tfp2);
     }
+    // 5c. the 64 / kWtvLanes voxels of one wavefront share one quad range:
+    //     voxel i's quad q (lane q % kWtvLanes, trip q / kWtvLanes) goes to
+    //     slot (q / kWtvLanes) * 64 + i * kWtvLanes + q % kWtvLanes, so each
+    //     load instruction of the wavefront reads 64 consecutive quads.  The
+    //     range lasts as many trips as the group's longest list; the rest is
+    //     padding (delta 0, weight 0).  tgs: the groups' entry offsets.
+    std::vector<int32_t> tgs{0};
+    {
+      constexpr int VPW = 64 / kWtvLanes;
+      const long nt = (long)tg.size(), ngrp = (nt + VPW - 1) / VPW;
+      for (long g = 0; g < ngrp; ++g) {
+        long trips = 0;
+        for (long k = g * VPW; k < std::min(nt, (g + 1) * VPW); ++k)
+          trips = std::max(trips, ((long)(trp[k + 1] - trp[k]) / 4 + kWtvLanes - 1) / kWtvLanes);
+        DR_REQUIRE((long)tgs.back() + trips * 256 < (1L << 31), DR_ELOGIC, "W stencil: too many transposed entries");
+        tgs.push_back((int32_t)(tgs.back() + trips * 256));
+      }
+      std::vector<uint16_t> tdg(tgs.back(), 0);
+      std::vector<T> twg(tgs.back(), (T)0);
+      for (long k = 0; k < nt; ++k) {
+        const long g = k / VPW, i = k % VPW, nq = (trp[k + 1] - trp[k]) / 4;
+        for (long q = 0; q < nq; ++q) {
+          const long dst = 4 * (tgs[g] / 4 + (q / kWtvLanes) * 64 + i * kWtvLanes + q % kWtvLanes);
+          for (int e = 0; e < 4; ++e) {
+            tdg[dst + e] = tdall[trp[k] + 4 * q + e];
+            twg[dst + e] = twall[trp[k] + 4 * q + e];
+          }
+        }
+      }
+      tdall.swap(tdg);
+      twall.swap(twg);
+    }
     // 6. every device access of the four kernels stays inside its buffer
     {
       long fmin = 0, fmax = 0;
@@ -1134,12 +1166,17 @@ struct Chain : public ChainBase {
         DR_REQUIRE(j >= 0 && j < nreg, DR_ELOGIC, "W stencil: boundary entry outside the region");
       }
     }
-    for (size_t k = 0; k < tg.size(); ++k) {
+    for (size_t k = 0; k < tg.size(); ++k)
       DR_REQUIRE(tg[k] >= 0 && tg[k] < nreg, DR_ELOGIC, "W stencil: touched voxel outside the region");
-      for (int32_t t = trp[k]; t < trp[k + 1]; ++t)
-        DR_REQUIRE((long)tbase[k] + tdall[t] < std::max<long>((long)brow.size(), 1), DR_ELOGIC,
-                   "W stencil: transposed entry outside the boundary rows");
-    }
+    DR_REQUIRE((long)tgs.size() == ((long)tg.size() + 64 / kWtvLanes - 1) / (64 / kWtvLanes) + 1, DR_ELOGIC,
+               "W stencil: transposed group offsets");
+    for (size_t g = 0; g + 1 < tgs.size(); ++g)
+      for (int32_t t = tgs[g]; t < tgs[g + 1]; ++t) {
+        const size_t k = g * (64 / kWtvLanes) + ((t - tgs[g]) / 4 % 64) / kWtvLanes;  // the voxel reading t
+        if (k < tg.size())
+          DR_REQUIRE((long)tbase[k] + tdall[t] < std::max<long>((long)brow.size(), 1), DR_ELOGIC,
+                     "W stencil: transposed entry outside the boundary rows");
+      }
     for (size_t k = 0; k < mrow.size(); ++k) {
       if (mrow[k] < 0) continue;
       const int p = mpar[k / rpb];
@@ -1192,7 +1229,7 @@ struct Chain : public ChainBase {
     upload(ws.bd, bd);
     upload(ws.bw, bw);
     upload(ws.tg, tg);
-    upload(ws.trp, trp);
+    upload(ws.trp, tgs);
     upload(ws.td, tdall);
     upload(ws.tw, twall);
     upload(ws.frow, frow);
