@@ -143,7 +143,9 @@ template <typename T> struct MidLeapfrog {
   struct State { cx<T> x, p; T il; };
   __device__ __forceinline__ State fetch(int k, int b) const {
     const long i = (long)k * sk + b;
-    return State{nt_ld(X + i), nt_ld(p + i), __builtin_nontemporal_load(invlam + i)};
+    // 1/lambda keeps cached loads: a 16-column tile reads half of each of its
+    // 128-byte lines, and the neighbouring tile the other half
+    return State{nt_ld(X + i), nt_ld(p + i), invlam[i]};
   }
   __device__ __forceinline__ cx<T> apply(int k, int b, cx<T> v, const State& s) {
     const long i = (long)k * sk + b;
