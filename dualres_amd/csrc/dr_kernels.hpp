@@ -311,10 +311,10 @@ template <typename T> struct LoadNoise {
   __device__ __forceinline__ void bind() {
     if (sc) stream = sc->stream[which];
   }
+  // normals in the chain's precision
   __device__ __forceinline__ cx<T> load2(int row, int n) {
-    double z0, z1;
-    normal_pair(seed, stream, (uint64_t)row * (uint64_t)M0h + (uint64_t)n, z0, z1);
-    const T a = (T)z0, b = (T)z1;
+    T a, b;
+    normal_pair(seed, stream, (uint64_t)row * (uint64_t)M0h + (uint64_t)n, a, b);
     acc += (double)a * (double)a + (double)b * (double)b;
     return mk<T>(a, b);
   }
