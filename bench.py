@@ -75,6 +75,44 @@ def run_chain(c, p, steps, warmup, seed=8675309, captured=False):
             "mh_rate": r["mh_rate"], "eps": r["eps"]}
 
 
+def precision_study(p, seeds=(1, 2, 3), burnin=200, nsave=200):
+    """FP32 against FP64 posterior means at the 1 mm workload over longer
+    chains: for each seed an FP32 and an FP64 fit (burnin + nsave iterations,
+    every post-burn-in iteration saved), then the relative differences of
+    their posterior means at the data voxels: FP32 vs FP64 on the same seed,
+    and FP64 vs FP64 (FP32 vs FP32) on different seeds, the Monte-Carlo
+    scale a precision difference has to be compared with."""
+    from dualres_amd.sampler import fit_native
+    means, stats = {}, {}
+    for s in seeds:
+        for prec in ("fp32", "fp64"):
+            c = make_chain(p, prec, "compat", seed=1000 + s)
+            r = fit_native(c, p["Ys"].size, THETA[0], burnin=burnin, nsave=nsave, thin=1, L=L_STEPS, seed=s)
+            s1, _, _, ns = c.summary_read()
+            c.close()
+            means[(prec, s)] = s1 / max(ns, 1)
+            stats[f"{prec}/{s}"] = {"mh_rate": round(r["mh_rate"], 4), "eps": round(r["eps"], 5), "saved": int(ns)}
+            log(f"[bench] precision study: {prec} seed {s}: mh {r['mh_rate']:.2f}, eps {r['eps']:.4g}, {ns} saved")
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    same_seed = [rel(means[("fp32", s)], means[("fp64", s)]) for s in seeds]
+    seed_pairs = [(a, b) for i, a in enumerate(seeds) for b in seeds[i + 1:]]
+    fp64_seeds = [rel(means[("fp64", a)], means[("fp64", b)]) for a, b in seed_pairs]
+    fp32_seeds = [rel(means[("fp32", a)], means[("fp32", b)]) for a, b in seed_pairs]
+    m32 = sum(means[("fp32", s)] for s in seeds) / len(seeds)
+    m64 = sum(means[("fp64", s)] for s in seeds) / len(seeds)
+    out = {"burnin": burnin, "nsave": nsave, "seeds": list(seeds), "chains": stats,
+           "fp32_vs_fp64_same_seed": [round(x, 5) for x in same_seed],
+           "fp64_vs_fp64_other_seed": [round(x, 5) for x in fp64_seeds],
+           "fp32_vs_fp32_other_seed": [round(x, 5) for x in fp32_seeds],
+           "seed_averaged_fp32_vs_fp64": round(rel(m32, m64), 5)}
+    log(f"[bench] precision study: fp32 vs fp64 same seed {out['fp32_vs_fp64_same_seed']}, fp64 other seed "
+        f"{out['fp64_vs_fp64_other_seed']}, fp32 other seed {out['fp32_vs_fp32_other_seed']}, "
+        f"seed-averaged fp32 vs fp64 {out['seed_averaged_fp32_vs_fp64']}")
+    return out
+
+
 def kernel_bytes(name, p, prec_bytes, c):
     """Algorithmic HBM bytes of one launch of the named kernel (read once + write once).
 
@@ -661,6 +699,8 @@ def main():
     ap.add_argument("--cpu-study", action="store_true",
                     help="time only the CPU restatement (float32 and float64, all and 80%% of the threads, OpenMP W; "
                          "medians of 3 iterations) and print its JSON")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="FP32 vs FP64 posterior means over 200 + 200 iterations on three seeds; print its JSON")
     ap.add_argument("--config1-only", action="store_true",
                     help="run only the 2 mm single-resolution fit on the GPU and the CPU restatement's 400-iteration "
                          "chain at the same workload, and print their JSON")
@@ -683,6 +723,9 @@ def main():
     p = build_problem()
     if a.cpu_study:
         print(json.dumps(cpu_study(p, eps=0.54)), flush=True)
+        return
+    if a.precision_study:
+        print(json.dumps(precision_study(p)), flush=True)
         return
     if a.mce_only:
         print(json.dumps(mce_leg(a.mce_images)), flush=True)
