@@ -392,6 +392,13 @@ __device__ __forceinline__ void bind_scalars(F& f) {
   if constexpr (HasBind<F>::value) f.bind();
 }
 
+// An epilogue with a State type and fetch(o, k, b) / store(o, k, b, v, state)
+// gets the state of 4 modes per thread fetched before they are updated.
+template <class F, class = void>
+struct HasFetch : std::false_type {};
+template <class F>
+struct HasFetch<F, std::void_t<typename F::State>> : std::true_type {};
+
 template <typename T, bool INV, class Pro, class Epi, int EPT = kEptStrided, int MC = 0>
 __global__ void __launch_bounds__(kThreads)
 k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi) {
@@ -413,9 +420,25 @@ k_strided(AxisFFT<T> ax, int NB, int k_in, int k_out, int tbs, Pro pro, Epi epi)
   }
   __syncthreads();
   fft_run<T, INV, EPT, MC, 1>(lds, TB, tbs, ax);
-  for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
-    const int b = idx & (TB - 1), k = idx >> tbs;
-    if (b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b]);
+  if constexpr (HasFetch<Epi>::value) {  // same modes, same order per thread
+    for (int idx0 = threadIdx.x; idx0 < k_out * TB; idx0 += 4 * kThreads) {
+      typename Epi::State st[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = idx0 + u * kThreads, b = idx & (TB - 1), k = idx >> tbs;
+        if (idx < k_out * TB && b0 + b < NB) st[u] = epi.fetch(o, k, b0 + b);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = idx0 + u * kThreads, b = idx & (TB - 1), k = idx >> tbs;
+        if (idx < k_out * TB && b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b], st[u]);
+      }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < k_out * TB; idx += kThreads) {
+      const int b = idx & (TB - 1), k = idx >> tbs;
+      if (b0 + b < NB) epi.store(o, k, b0 + b, lds[k * TB + b]);
+    }
   }
   const int blk = blockIdx.y * gridDim.x + blockIdx.x;
   pro.finish(red, blk);

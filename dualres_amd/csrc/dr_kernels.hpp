@@ -76,16 +76,29 @@ template <typename T> struct EpiMomentum {
       sig = sc->sig0;
     }
   }
-  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) {
+  // the state one mode's update reads (k_strided fetches several modes'
+  // state before updating them)
+  struct State { T il; cx<T> x, p, xi, pi; };
+  __device__ __forceinline__ State fetch(int, int k, int b) const {
     const long i = (long)k * sk + b;
-    const T il = invlam[i];
-    const cx<T> x = X[i];
-    const cx<T> pn = p[i] + (v - x * il) * coef;
+    State s{invlam[i], X[i], p[i], mk<T>(0, 0), mk<T>(0, 0)};
+    if (pim) {
+      s.xi = Xi[i];
+      s.pi = pim[i];
+    }
+    return s;
+  }
+  __device__ __forceinline__ void store(int o, int k, int b, cx<T> v) { store(o, k, b, v, fetch(o, k, b)); }
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v, const State& s) {
+    const long i = (long)k * sk + b;
+    const T il = s.il;
+    const cx<T> x = s.x;
+    const cx<T> pn = s.p + (v - x * il) * coef;
     p[i] = pn;
     cx<T> xi = mk<T>(0, 0), pin = mk<T>(0, 0);
     if (pim) {
-      xi = Xi[i];
-      pin = pim[i] - xi * (il * coef);
+      xi = s.xi;
+      pin = s.pi - xi * (il * coef);
       pim[i] = pin;
     }
     if (do_red) {
@@ -229,10 +242,15 @@ template <typename T> struct EpiNoiseMomentum {
   __device__ __forceinline__ void bind() {
     if (sc) sig = sc->sig0;
   }
-  __device__ __forceinline__ void store(int, int k, int b, cx<T> v) {
+  struct State { T il; };
+  __device__ __forceinline__ State fetch(int, int k, int b) const { return State{invlam[(long)k * sk + b]}; }
+  __device__ __forceinline__ void store(int o, int k, int b, cx<T> v) { store(o, k, b, v, fetch(o, k, b)); }
+  // sqrt in T: the correctly rounded float square root equals the float
+  // rounding of the double one
+  __device__ __forceinline__ void store(int, int k, int b, cx<T> v, const State& s) {
     const long i = (long)k * sk + b;
-    const T il = invlam[i];
-    const cx<T> pn = il > (T)0 ? v * (T)sqrt((double)(il + sig)) : mk<T>(0, 0);
+    const T il = s.il;
+    const cx<T> pn = il > (T)0 ? v * sqrt(il + sig) : mk<T>(0, 0);
     p[i] = pn;
     if (part) {
       const double e = il > (T)0 ? 1.0 / (double)(il + sig) * ((double)pn.x * pn.x + (double)pn.y * pn.y)
