@@ -149,8 +149,9 @@ def kernel_bytes(name, p, prec_bytes, c):
         if name == "W_mu":   # interior: row words, Ys, v, stencil, R; boundary: row, rp, base, Ys, v; entries
             return (ni * (12 + 2 * s) + nd * (4 + s) + n_h * s) + (nb * (12 + 2 * s) + eb * (2 + s))
         if name == "Wt_v":   # interior: per tile its lattice box and voxels (mask in, W^T v out), the stencil;
-            #                   boundary: voxel words (g, rp, base, far rp), entries, boundary v
-            return (ntile * (12 + ncell * s + tvox * (1 + s)) + nd * (4 + s)) + (nt * (16 + s) + eb * (2 + s) + nb * s)
+            #                   boundary: voxel words (g, base, far rp; the quad ranges are per 16
+            #                   voxels), entries, boundary v
+            return (ntile * (12 + ncell * s + tvox * (1 + s)) + nd * (4 + s)) + (nt * (12 + s) + eb * (2 + s) + nb * s)
     if name == "W_mu":
         return nnz * (4 + s) + (n_s + 1) * 4 + 2 * n_s * s + n_h * s
     if name == "Wt_v":
