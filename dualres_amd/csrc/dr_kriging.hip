@@ -219,9 +219,9 @@ __global__ void k_assemble(const SysDesc* __restrict__ sd, int np, const float* 
 typedef double dr_v4f64 __attribute__((ext_vector_type(4)));
 typedef double Tile[NB][NB + 1];
 
-struct Quad {  // this thread's place in the tile products
+struct TileLane {  // this thread's place in the tile products
   int r0, c0, li, lk;
-  __device__ Quad() {
+  __device__ TileLane() {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     r0 = (wave >> 1) * 32;
     c0 = (wave & 1) * 32;
@@ -234,7 +234,7 @@ struct Quad {  // this thread's place in the tile products
 };
 
 // acc += X Y^T over 64 columns, X and Y row-major 64x64 tiles in LDS
-__device__ __forceinline__ void mfma_xyt(const Tile& X, const Tile& Y, const Quad& q, dr_v4f64 acc[2][2]) {
+__device__ __forceinline__ void mfma_xyt(const Tile& X, const Tile& Y, const TileLane& q, dr_v4f64 acc[2][2]) {
 #pragma unroll 4
   for (int kk = 0; kk < NB; kk += 4) {
     double av[2], bv[2];
@@ -263,7 +263,7 @@ __device__ __forceinline__ void zero_acc(dr_v4f64 acc[2][2]) {
 // into registers while the matrix cores work on panel l from LDS.  Returns
 // with every thread past its last read of Pi and Pj.
 __device__ __forceinline__ void col_update(const double* As, int np, int BI, int BJ, int j, Tile& Pi, Tile& Pj,
-                                           const Quad& q, double cv[2][2][4]) {
+                                           const TileLane& q, double cv[2][2][4]) {
   const int lc = threadIdx.x & 63, lr = threadIdx.x >> 6;  // loads: column lc of rows lr + 4 r
   double gi[16], gj[16];
   auto fetch = [&](int l) {
@@ -302,7 +302,7 @@ __device__ __forceinline__ void col_update(const double* As, int np, int BI, int
   __syncthreads();
 }
 
-__device__ __forceinline__ void cv_to_lds(const double cv[2][2][4], const Quad& q, Tile& T) {
+__device__ __forceinline__ void cv_to_lds(const double cv[2][2][4], const TileLane& q, Tile& T) {
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -387,7 +387,7 @@ __device__ __forceinline__ void lds_to_tile(const Tile& T, double* As, int np, i
 __global__ void __launch_bounds__(256) k_chol_diag(double* A, int np, int j, int* fail) {
   __shared__ Tile Pi, Pj;
   double* As = A + (long)blockIdx.x * np * np;
-  const Quad q;
+  const TileLane q;
   double cv[2][2][4];
   col_update(As, np, j * NB, j * NB, j, Pi, Pj, q, cv);
   cv_to_lds(cv, q, Pi);
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* A, int np, int j) {
   __shared__ Tile Pi, Pj;
   double* As = A + (long)blockIdx.x * np * np;
   const int BI = (j + 1 + blockIdx.y) * NB, o = j * NB;
-  const Quad q;
+  const TileLane q;
   double cv[2][2][4];
   col_update(As, np, BI, o, j, Pi, Pj, q, cv);
   cv_to_lds(cv, q, Pi);
@@ -550,7 +550,7 @@ __global__ void __launch_bounds__(256) k_gemm_lu_mfma(double* A, int np, int k, 
     Pj[l][i] = As[(long)(o + i) * np + BJ + l];  // U12^T
   }
   __syncthreads();
-  const Quad q;
+  const TileLane q;
   dr_v4f64 acc[2][2];
   zero_acc(acc);
   mfma_xyt(Pi, Pj, q, acc);
