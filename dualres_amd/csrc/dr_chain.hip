@@ -518,6 +518,19 @@ struct Chain : public ChainBase {
     strided<T, false>(2, 1, M[1] * H0, M[2], M[2], LoadS<T>{{}, T1.p, 0, (long)M[1] * H0}, epi, slot);
   }
 
+  // the spectral momentum drawn directly (k_momentum_direct), E0 sums to slot
+  void momentum_direct(uint64_t stream_id, cx<T>* dst, int slot) {
+    DR_REQUIRE(M[0] % 2 == 0 && M[1] % 2 == 0 && M[2] % 2 == 0, DR_ELOGIC, "momentum draw: odd torus size");
+    const long Nh = (long)H0 * M[1] * M[2];
+    const int g = (int)std::min<long>(cdiv(Nh, kThreads), 4096);
+    note_blocks(slot, g);
+    const double nN = (double)M[0] * M[1] * M[2];
+    hipLaunchKernelGGL(k_momentum_direct<T>, dim3(g), dim3(kThreads), 0, stream, Nh, H0, M[1], M[2], cfg.seed,
+                       stream_id, (int)(stream_id & 1), (T)sigma[0], invlam.p, dst, pslot(slot), gsc,
+                       (T)std::sqrt(0.5 * nN), (T)std::sqrt(nN));
+    DR_HIP(hipGetLastError());
+  }
+
   void finalize() {
     FinalizeArgs a{};
     a.nslot = NSLOT;
@@ -1535,13 +1548,23 @@ struct Chain : public ChainBase {
   void device_iteration(double eps, int L, uint64_t it, const double* zr, const double* zi) {
     const long sk = (long)M[1] * H0;
     clear_slots();
-    if (zr) upload_real(zr, zbuf);
-    noise_r2c(4 * it + 0, zr ? zbuf.p : nullptr, S_E0,
-              EpiNoiseMomentum<T>{p.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0), 0.0});
+    // injected real-space normals go through the transforms; the product
+    // path draws the spectral momentum directly
+    if (zr) {
+      upload_real(zr, zbuf);
+      noise_r2c(4 * it + 0, zbuf.p, S_E0,
+                EpiNoiseMomentum<T>{p.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0), 0.0});
+    } else {
+      momentum_direct(4 * it + 0, p.p, S_E0);
+    }
     if (compat()) {
-      if (zr) upload_real(zi, zbuf);
-      noise_r2c(4 * it + 1, zr ? zbuf.p : nullptr, S_E0I,
-                EpiNoiseMomentum<T>{pim.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0I), 0.0});
+      if (zr) {
+        upload_real(zi, zbuf);
+        noise_r2c(4 * it + 1, zbuf.p, S_E0I,
+                  EpiNoiseMomentum<T>{pim.p, invlam.p, sk, (T)sigma[0], gsc, H0, pslot(S_E0I), 0.0});
+      } else {
+        momentum_direct(4 * it + 1, pim.p, S_E0I);
+      }
     }
     leapfrog_core(eps, L);
     finalize();

@@ -263,6 +263,52 @@ template <typename T> struct EpiNoiseMomentum {
   }
 };
 
+// The momentum draw in one pass over the half spectrum.  For a real field z
+// of iid N(0, 1) on the N-point torus, F(z) is Hermitian white noise: every
+// mode k with a distinct partner -k has Re and Im iid N(0, N / 2) and F(z)(-k)
+// = conj F(z)(k); a self-conjugate mode (k = -k) is real N(0, N).  So the
+// spectral momentum sqrt(lambda_m) F(z) of EpiNoiseMomentum is drawn here
+// directly, without the three transforms: one Philox pair per mode, keyed by
+// the mode's index, and in the planes kx = 0 and kx = M0 / 2 (where the half
+// spectrum holds both k and -k) by the smaller index of the pair, the other
+// taking the conjugate.  E0 is summed as EpiNoiseMomentum sums it.  Even M0,
+// M1, M2.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_momentum_direct(long Nh, int H0, int M1, int M2, uint64_t seed, uint64_t stream, int which, T sig,
+                  const T* __restrict__ invlam, cx<T>* __restrict__ p, double* part, const StepScalars<T>* sc,
+                  T sd_pair, T sd_self) {
+  __shared__ double red[kThreads];
+  if (sc) {
+    stream = sc->stream[which];
+    sig = sc->sig0;
+  }
+  const long plane = (long)H0 * M1;
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < Nh; i += (long)gridDim.x * kThreads) {
+    const int kx = (int)(i % H0);
+    long key = i;
+    bool self = false, cj = false;
+    if (kx == 0 || kx == H0 - 1) {
+      const int ky = (int)((i / H0) % M1), kz = (int)(i / plane);
+      const long ip = (long)((M2 - kz) % M2) * plane + (long)((M1 - ky) % M1) * H0 + kx;
+      self = ip == i;
+      cj = ip < i;
+      key = cj ? ip : i;
+    }
+    T a, b;
+    normal_pair(seed, stream, (uint64_t)key, a, b);
+    const cx<T> xi = self ? mk<T>(a * sd_self, (T)0) : mk<T>(a * sd_pair, (cj ? -b : b) * sd_pair);
+    const T il = invlam[i];
+    const cx<T> pn = il > (T)0 ? xi * sqrt(il + sig) : mk<T>(0, 0);
+    p[i] = pn;
+    const double e = il > (T)0 ? 1.0 / (double)(il + sig) * ((double)pn.x * pn.x + (double)pn.y * pn.y)
+                               : (double)xi.x * xi.x + (double)xi.y * xi.y;
+    acc += herm_w(kx, H0) * e;
+  }
+  red_write(acc, red, part, blockIdx.x);
+}
+
 // initial state mu = sqrt(lambda) F(z) / 10, 0 on Z (MultiResParameters.h:752-755)
 template <typename T> struct EpiInitMu : NoRed {
   cx<T>* X; const double* lam; long sk;

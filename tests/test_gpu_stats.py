@@ -3,8 +3,9 @@
 The product path draws its momentum and initial state from a counter-based
 Philox4x32-10 stream with Box-Muller normals on the device (the reference
 draws them from std::mt19937, MultiResParameters.h:743-759, 783-815, so the
-values cannot match draw for draw).  These tests check what the reference's
-algorithm fixes regardless of the generator:
+values cannot match draw for draw); the momentum's are drawn directly as the
+Hermitian white-noise spectrum F(z) of a white field z.  These tests check
+what the reference's algorithm fixes regardless of the generator:
 
 * E0, which the engine sums from the fresh momentum as the end point's
   kinetic energy is summed, equals -1/2 sum |z|^2 of the normals recovered
