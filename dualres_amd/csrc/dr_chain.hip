@@ -522,7 +522,8 @@ struct Chain : public ChainBase {
   void momentum_direct(uint64_t stream_id, cx<T>* dst, int slot) {
     DR_REQUIRE(M[0] % 2 == 0 && M[1] % 2 == 0 && M[2] % 2 == 0, DR_ELOGIC, "momentum draw: odd torus size");
     const long Nh = (long)H0 * M[1] * M[2];
-    const int g = (int)std::min<long>(cdiv(Nh, kThreads), 4096);
+    DR_REQUIRE(Nh < (1L << 31), DR_ELOGIC, "momentum draw: half spectrum too large for 32-bit indices");
+    const int g = (int)std::min<long>(cdiv(Nh / 2, kThreads), 4096);  // two modes per thread
     note_blocks(slot, g);
     const double nN = (double)M[0] * M[1] * M[2];
     hipLaunchKernelGGL(k_momentum_direct<T>, dim3(g), dim3(kThreads), 0, stream, Nh, H0, M[1], M[2], cfg.seed,

@@ -129,24 +129,34 @@ __host__ __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t str
   z1 = rad * s;
 }
 
-// The same pair in float32 arithmetic (24-bit uniforms strictly inside (0, 1),
-// so |z| < 5.9), for the float32 chain: float32 logarithm and sine/cosine
-// cost a fraction of the float64 ones.
-__host__ __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t stream, uint64_t pair, float& z0,
-                                                     float& z1) {
+// The Philox block of counter (pair, stream).
+__host__ __device__ __forceinline__ u4 philox_block(uint64_t seed, uint64_t stream, uint64_t pair) {
   u4 c;
   c.a = (uint32_t)pair;
   c.b = (uint32_t)(pair >> 32);
   c.c = (uint32_t)stream;
   c.d = (uint32_t)(stream >> 32);
-  const u4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const float u1 = ((float)(r.a >> 8) + 0.5f) * (1.0f / 16777216.0f);
-  const float u2 = ((float)(r.c >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Box-Muller in float32 arithmetic from two 32-bit words (24-bit uniforms
+// strictly inside (0, 1), so |z| < 5.9): float32 logarithm and sine/cosine
+// cost a fraction of the float64 ones.
+__host__ __device__ __forceinline__ void box_muller_f(uint32_t w1, uint32_t w2, float& z0, float& z1) {
+  const float u1 = ((float)(w1 >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float u2 = ((float)(w2 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float rad = sqrtf(-2.0f * logf(u1));
   float s, co;
   sincospif(2.0f * u2, &s, &co);
   z0 = rad * co;
   z1 = rad * s;
+}
+
+// The pair in float32 for the float32 chain, from words a and c of the block.
+__host__ __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t stream, uint64_t pair, float& z0,
+                                                     float& z1) {
+  const u4 r = philox_block(seed, stream, pair);
+  box_muller_f(r.a, r.c, z0, z1);
 }
 
 // ---------------------------------------------------------------- separately rounded float ops

@@ -268,11 +268,31 @@ template <typename T> struct EpiNoiseMomentum {
 // mode k with a distinct partner -k has Re and Im iid N(0, N / 2) and F(z)(-k)
 // = conj F(z)(k); a self-conjugate mode (k = -k) is real N(0, N).  So the
 // spectral momentum sqrt(lambda_m) F(z) of EpiNoiseMomentum is drawn here
-// directly, without the three transforms: one Philox pair per mode, keyed by
+// directly, without the three transforms: one normal pair per mode, keyed by
 // the mode's index, and in the planes kx = 0 and kx = M0 / 2 (where the half
 // spectrum holds both k and -k) by the smaller index of the pair, the other
-// taking the conjugate.  E0 is summed as EpiNoiseMomentum sums it.  Even M0,
-// M1, M2.
+// taking the conjugate.  Float64: the pair of key q comes from Philox block q
+// (53-bit uniforms from its four words).  Float32: from block q / 2, words
+// (a, c) for even q and (b, d) for odd q, so a thread's two consecutive modes
+// share one block.  E0 is summed as EpiNoiseMomentum sums it.  Even M0, M1,
+// M2 (Nh is even, below 2^31: the index arithmetic is 32-bit).
+struct ModeKey {
+  uint32_t key, kx;
+  bool self, cj;
+};
+__device__ __forceinline__ ModeKey mode_key(uint32_t i, uint32_t H0, uint32_t M1, uint32_t M2) {
+  const uint32_t row = i / H0, kx = i - row * H0;
+  ModeKey m{i, kx, false, false};
+  if (kx == 0 || kx == H0 - 1) {
+    const uint32_t kz = row / M1, ky = row - kz * M1;
+    const uint32_t ip = (((M2 - kz) % M2) * M1 + (M1 - ky) % M1) * H0 + kx;
+    m.self = ip == i;
+    m.cj = ip < i;
+    if (m.cj) m.key = ip;
+  }
+  return m;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_momentum_direct(long Nh, int H0, int M1, int M2, uint64_t seed, uint64_t stream, int which, T sig,
@@ -283,28 +303,33 @@ k_momentum_direct(long Nh, int H0, int M1, int M2, uint64_t seed, uint64_t strea
     stream = sc->stream[which];
     sig = sc->sig0;
   }
-  const long plane = (long)H0 * M1;
   double acc = 0.0;
-  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < Nh; i += (long)gridDim.x * kThreads) {
-    const int kx = (int)(i % H0);
-    long key = i;
-    bool self = false, cj = false;
-    if (kx == 0 || kx == H0 - 1) {
-      const int ky = (int)((i / H0) % M1), kz = (int)(i / plane);
-      const long ip = (long)((M2 - kz) % M2) * plane + (long)((M1 - ky) % M1) * H0 + kx;
-      self = ip == i;
-      cj = ip < i;
-      key = cj ? ip : i;
+  for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; 2 * (long)t < Nh; t += gridDim.x * kThreads) {
+    ModeKey m[2];
+    T a[2], b[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) m[u] = mode_key(2 * t + u, (uint32_t)H0, (uint32_t)M1, (uint32_t)M2);
+    if constexpr (sizeof(T) == 4) {
+      const u4 r0 = philox_block(seed, stream, (uint64_t)(m[0].key >> 1));
+      const u4 r1 = (m[1].key >> 1) == (m[0].key >> 1) ? r0 : philox_block(seed, stream, (uint64_t)(m[1].key >> 1));
+      box_muller_f((m[0].key & 1) ? r0.b : r0.a, (m[0].key & 1) ? r0.d : r0.c, a[0], b[0]);
+      box_muller_f((m[1].key & 1) ? r1.b : r1.a, (m[1].key & 1) ? r1.d : r1.c, a[1], b[1]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) normal_pair(seed, stream, (uint64_t)m[u].key, a[u], b[u]);
     }
-    T a, b;
-    normal_pair(seed, stream, (uint64_t)key, a, b);
-    const cx<T> xi = self ? mk<T>(a * sd_self, (T)0) : mk<T>(a * sd_pair, (cj ? -b : b) * sd_pair);
-    const T il = invlam[i];
-    const cx<T> pn = il > (T)0 ? xi * sqrt(il + sig) : mk<T>(0, 0);
-    p[i] = pn;
-    const double e = il > (T)0 ? 1.0 / (double)(il + sig) * ((double)pn.x * pn.x + (double)pn.y * pn.y)
-                               : (double)xi.x * xi.x + (double)xi.y * xi.y;
-    acc += herm_w(kx, H0) * e;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t i = 2 * t + u;
+      const cx<T> xi = m[u].self ? mk<T>(a[u] * sd_self, (T)0)
+                                 : mk<T>(a[u] * sd_pair, (m[u].cj ? -b[u] : b[u]) * sd_pair);
+      const T il = invlam[i];
+      const cx<T> pn = il > (T)0 ? xi * sqrt(il + sig) : mk<T>(0, 0);
+      p[i] = pn;
+      const double e = il > (T)0 ? 1.0 / (double)(il + sig) * ((double)pn.x * pn.x + (double)pn.y * pn.y)
+                                 : (double)xi.x * xi.x + (double)xi.y * xi.y;
+      acc += herm_w((int)m[u].kx, H0) * e;
+    }
   }
   red_write(acc, red, part, blockIdx.x);
 }
