@@ -894,7 +894,7 @@ struct Chain : public ChainBase {
     long ncw = 0;
     for (int p = 0; p < 4; ++p) ncw = std::max<long>(ncw, wstart[p + 1] - wstart[p]);
     if (ncw * (4 + 4 * (long)sizeof(T)) > 48 * 1024) return false;  // LDS window table of W mu
-    const int rpb = kThreads / 8;
+    const int rpb = kThreads / kWmuLanes;
     std::vector<int32_t> mrow, mbase, mlidx, mpar;
     for (int p = 0; p < 4; ++p) {
       const size_t s0 = mrow.size();
@@ -1287,7 +1287,7 @@ struct Chain : public ChainBase {
     DR_HIP(hipMemcpyAsync(Yg.p, yg.data(), nreg * sizeof(T), hipMemcpyHostToDevice, stream));
     DR_HIP(hipMemcpyAsync(mask.p, mk_.data(), nreg, hipMemcpyHostToDevice, stream));
     if (cfg.n_datasets == 2) {
-      const long need = cdiv(n_s, kThreads / 64) + cdiv(n_s * 8, kThreads) + 8;  // + parity-bucket padding
+      const long need = cdiv(n_s, kThreads / 64) + cdiv(n_s * kWmuLanes, kThreads) + 8;  // + parity-bucket padding
       if (need > pstride) {
         pstride = need;
         part.alloc((size_t)NSLOT * pstride);
@@ -1379,7 +1379,7 @@ struct Chain : public ChainBase {
     const int g1 = cdiv(n_s, kThreads / 64);
     const int g2 = cdiv(n_h, kThreads / 16);
     if (use_cls) {
-      const int gi = cdiv((long)ws.n_mslot * 8, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
+      const int gi = cdiv((long)ws.n_mslot * kWmuLanes, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
                 gf = cdiv(ws.n_far, kThreads / 64);
       if (slot >= 0) note_blocks(slot, gi + gb + gf);
       const int R0 = reg[0], R01 = reg[0] * reg[1];
