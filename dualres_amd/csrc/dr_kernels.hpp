@@ -595,8 +595,10 @@ __device__ __forceinline__ unsigned lo16(unsigned x) { return x & 0xffffu; }
 __device__ __forceinline__ unsigned hi16(unsigned x) { return x >> 16; }
 
 // lanes per voxel of the boundary W^T v part (the transposed entry lists are
-// interleaved in blocks of this many quads)
-constexpr int kWtvLanes = 4;
+// interleaved in blocks of this many quads).  Measured at 1 mm with the
+// wavefront-contiguous lists: W^T v 79 / 76.5 / 80 / 92 us with 1 / 2 / 4 / 8
+// lanes per voxel.
+constexpr int kWtvLanes = 2;
 
 // Argument bundles of the four parts of the two products.
 // bit patterns of T: weights read through them are wave-uniform loads the
