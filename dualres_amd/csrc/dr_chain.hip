@@ -995,7 +995,7 @@ struct Chain : public ChainBase {
         bd.push_back(0);
         bw.push_back((T)0);
       }
-      interleave_quads(bd, bw, brp.back(), (long)bw.size(), 64);
+      interleave_quads(bd, bw, brp.back(), (long)bw.size(), kWmuBndLanes);
       brow.push_back((int32_t)i);
       bbase.push_back((int32_t)base);
       brp.push_back((int32_t)bw.size());
@@ -1379,7 +1379,7 @@ struct Chain : public ChainBase {
     const int g1 = cdiv(n_s, kThreads / 64);
     const int g2 = cdiv(n_h, kThreads / 16);
     if (use_cls) {
-      const int gi = cdiv((long)ws.n_mslot * kWmuLanes, kThreads), gb = cdiv(ws.n_bnd, kThreads / 64),
+      const int gi = cdiv((long)ws.n_mslot * kWmuLanes, kThreads), gb = cdiv(ws.n_bnd, kThreads / kWmuBndLanes),
                 gf = cdiv(ws.n_far, kThreads / 64);
       if (slot >= 0) note_blocks(slot, gi + gb + gf);
       const int R0 = reg[0], R01 = reg[0] * reg[1];
