@@ -161,9 +161,10 @@ int dr_get_tau_sq_inv(dr_chain* c, double* tau_sq_inv);
  * of the current state. */
 int dr_mu_data_sumsq(dr_chain* c, double* out);
 
-/* One HMC iteration (MultiResParameters::update, :673-727) with momentum
- * normals from the device Philox stream and the host-drawn accept uniform
- * u_accept in [0,1).  Does not perform the Gibbs draws. */
+/* One HMC iteration (MultiResParameters::update, :673-727) with the
+ * momentum drawn from the device Philox stream (directly as the spectrum
+ * F(z) of a white field z, Hermitian white noise) and the host-drawn accept
+ * uniform u_accept in [0,1).  Does not perform the Gibbs draws. */
 int dr_update(dr_chain* c, double eps, int L, double u_accept, dr_update_result* res);
 /* Same with injected momentum normals z = z_re + i z_im on the full torus
  * (length N each, x fastest); z_im is ignored in DR_STATE_REAL. */
